@@ -1,0 +1,351 @@
+"""Synthetic unstructured meshes for the BES solver core, plus a Gmsh MSH2 writer.
+
+Every generator returns a :class:`MeshData`: node coordinates, volume elements
+(CCW triangles in 2D, positively oriented tetrahedra in 3D) and boundary faces
+with a boundary type, listed in the order a Gmsh MSH2 file would hold them
+(boundary elements first, then volume elements).  That order matters: the
+reference numbers faces "boundary faces in file order, then internal faces in
+discovery order" (Mesh.cpp:236-534), and the solver core reproduces it.
+
+The MSH2 subset written here is the one the reference reader accepts
+(Mesh.cpp:95-558): ``$PhysicalNames`` before ``$Nodes``, exactly two tags per
+element with the physical tag first, element types 1 (line), 2 (triangle) and
+4 (tetrahedron) only, contiguous node ids from 1, single-space separators.
+"""
+from __future__ import annotations
+
+import dataclasses
+import numpy as np
+
+# Boundary type codes, matching the reference enum BoundaryType
+# (Options.hpp:34-35): INTERNAL=1, SLIP_WALL=2, FREESTREAM=3.
+INTERNAL = 1
+SLIP_WALL = 2
+FREESTREAM = 3
+BOUNDARY_NAMES = {SLIP_WALL: "SLIP_WALL", FREESTREAM: "FREESTREAM"}
+
+
+@dataclasses.dataclass
+class MeshData:
+    ndim: int
+    nodes: np.ndarray        # [nnode, ndim] float64
+    cells: np.ndarray        # [ncell, ndim+1] int64, 0-based node ids
+    bfaces: np.ndarray       # [nbface, ndim] int64, 0-based node ids
+    btype: np.ndarray        # [nbface] int32 boundary type code
+
+    @property
+    def ncell(self) -> int:
+        return int(self.cells.shape[0])
+
+    @property
+    def nnode(self) -> int:
+        return int(self.nodes.shape[0])
+
+
+# --------------------------------------------------------------------------
+# helpers
+# --------------------------------------------------------------------------
+
+def _orient_tris(nodes, tris):
+    """Return triangles in counter-clockwise order (reference Q10 for 2D)."""
+    p = nodes[tris]
+    cross = ((p[:, 1, 0] - p[:, 0, 0]) * (p[:, 2, 1] - p[:, 0, 1])
+             - (p[:, 2, 0] - p[:, 0, 0]) * (p[:, 1, 1] - p[:, 0, 1]))
+    tris = tris.copy()
+    neg = cross < 0
+    tris[neg, 1], tris[neg, 2] = tris[neg, 2].copy(), tris[neg, 1].copy()
+    return tris
+
+
+def tet_signed_volume6(nodes, tets):
+    p = nodes[tets]
+    a = p[:, 1] - p[:, 0]
+    b = p[:, 2] - p[:, 0]
+    c = p[:, 3] - p[:, 0]
+    return np.einsum("ij,ij->i", a, np.cross(b, c))
+
+
+def _orient_tets(nodes, tets):
+    """Return tetrahedra with positive orientation (reference Q10 for 3D)."""
+    v = tet_signed_volume6(nodes, tets)
+    tets = tets.copy()
+    neg = v < 0
+    tets[neg, 2], tets[neg, 3] = tets[neg, 3].copy(), tets[neg, 2].copy()
+    return tets
+
+
+def boundary_faces_of(cells: np.ndarray, ndim: int) -> np.ndarray:
+    """Faces (as node tuples) that belong to exactly one cell."""
+    if ndim == 2:
+        loc = [(0, 1), (1, 2), (2, 0)]
+    else:
+        loc = [(0, 2, 1), (1, 2, 3), (0, 3, 2), (0, 1, 3)]
+    faces = np.concatenate([cells[:, list(l)] for l in loc], axis=0)
+    key = np.sort(faces, axis=1)
+    _, inv, cnt = np.unique(key, axis=0, return_inverse=True, return_counts=True)
+    inv = inv.reshape(-1)
+    once = cnt[inv] == 1
+    return faces[once]
+
+
+# --------------------------------------------------------------------------
+# 2D channel with a circular-arc bump (config: 200x100 quads split in 40k tris)
+# --------------------------------------------------------------------------
+
+def bump_lower_wall(x, thickness=0.1, x0=1.0, x1=2.0):
+    """Circular arc of the given thickness on x in [x0, x1], zero elsewhere."""
+    half = 0.5 * (x1 - x0)
+    radius = (half * half + thickness * thickness) / (2.0 * thickness)
+    xc = 0.5 * (x0 + x1)
+    y = np.sqrt(np.maximum(radius * radius - (x - xc) ** 2, 0.0)) - (radius - thickness)
+    return np.where((x > x0) & (x < x1), np.maximum(y, 0.0), 0.0)
+
+
+def bump_channel(nx=200, ny=100, seed=20261202, jitter=0.1, length=3.0, height=1.0):
+    """2D channel [0,length]x[0,height] with a 10%-thick circular-arc bump.
+
+    (nx+1)x(ny+1) nodes, algebraic spacing from the lower wall to the top,
+    each quad split on a seeded random diagonal, interior nodes jittered by
+    ``jitter`` times the local spacing.  Lower and upper walls are slip walls,
+    inlet and outlet are freestream boundaries.
+    """
+    rng_diag = np.random.default_rng(seed)
+    rng_jit = np.random.default_rng(seed + 1)
+    xs = np.linspace(0.0, length, nx + 1)
+    yb = bump_lower_wall(xs)
+    eta = np.linspace(0.0, 1.0, ny + 1)
+    X = np.repeat(xs[:, None], ny + 1, axis=1)
+    Y = yb[:, None] + eta[None, :] * (height - yb[:, None])
+    dx = length / nx
+    dy = (height - yb[:, None]) / ny * np.ones((1, ny + 1))
+    jx = rng_jit.uniform(-jitter, jitter, size=X.shape) * dx
+    jy = rng_jit.uniform(-jitter, jitter, size=X.shape) * dy
+    interior = np.zeros_like(X, dtype=bool)
+    interior[1:-1, 1:-1] = True
+    X = X + np.where(interior, jx, 0.0)
+    Y = Y + np.where(interior, jy, 0.0)
+    nodes = np.stack([X.ravel(), Y.ravel()], axis=1)
+
+    def nid(i, j):
+        return i * (ny + 1) + j
+
+    I, J = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
+    I = I.ravel()
+    J = J.ravel()
+    a = nid(I, J)
+    b = nid(I + 1, J)
+    c = nid(I + 1, J + 1)
+    d = nid(I, J + 1)
+    diag = rng_diag.random(I.shape[0]) < 0.5
+    t1 = np.where(diag[:, None], np.stack([a, b, c], 1), np.stack([a, b, d], 1))
+    t2 = np.where(diag[:, None], np.stack([a, c, d], 1), np.stack([b, c, d], 1))
+    tris = np.stack([t1, t2], axis=1).reshape(-1, 3)
+    tris = _orient_tris(nodes, tris)
+
+    # boundary edges in a fixed order: bottom, right, top, left
+    bottom = np.stack([nid(np.arange(nx), 0), nid(np.arange(1, nx + 1), 0)], 1)
+    right = np.stack([nid(nx, np.arange(ny)), nid(nx, np.arange(1, ny + 1))], 1)
+    top = np.stack([nid(np.arange(nx, 0, -1), ny), nid(np.arange(nx - 1, -1, -1), ny)], 1)
+    left = np.stack([nid(0, np.arange(ny, 0, -1)), nid(0, np.arange(ny - 1, -1, -1))], 1)
+    bfaces = np.concatenate([bottom, right, top, left], axis=0)
+    btype = np.concatenate([
+        np.full(nx, SLIP_WALL), np.full(ny, FREESTREAM),
+        np.full(nx, SLIP_WALL), np.full(ny, FREESTREAM)]).astype(np.int32)
+    return MeshData(2, nodes, tris.astype(np.int64), bfaces.astype(np.int64), btype)
+
+
+# --------------------------------------------------------------------------
+# 3D box of Kuhn-split hexahedra (tests and synthetic throughput meshes)
+# --------------------------------------------------------------------------
+
+_KUHN = [(0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0)]
+
+
+def kuhn_tets(ni, nj, nk, node_id):
+    """Split each hexahedron of an ni x nj x nk lattice into 6 Kuhn tetrahedra.
+
+    ``node_id(i, j, k)`` maps lattice corners (arrays) to node ids.  The split
+    along the (0,0,0)-(1,1,1) diagonal of every hex is conforming across hex
+    faces, so the result is a valid tetrahedral mesh.
+    """
+    I, J, K = np.meshgrid(np.arange(ni), np.arange(nj), np.arange(nk), indexing="ij")
+    I = I.ravel(); J = J.ravel(); K = K.ravel()
+    base = np.stack([I, J, K], 1)
+    tets = []
+    for perm in _KUHN:
+        p = base.copy()
+        verts = [node_id(p[:, 0], p[:, 1], p[:, 2])]
+        for ax in perm:
+            p = p.copy()
+            p[:, ax] += 1
+            verts.append(node_id(p[:, 0], p[:, 1], p[:, 2]))
+        tets.append(np.stack(verts, 1))
+    return np.stack(tets, 1).reshape(-1, 4)
+
+
+def box_tets(ni=4, nj=3, nk=3, seed=7, jitter=0.15, extent=(1.0, 1.0, 1.0),
+             slip_faces=("zmin",)):
+    """Tetrahedral box mesh with jittered interior nodes.
+
+    Faces named in ``slip_faces`` (xmin/xmax/ymin/ymax/zmin/zmax) are slip
+    walls, the rest are freestream boundaries.
+    """
+    rng = np.random.default_rng(seed)
+    xs = np.linspace(0.0, extent[0], ni + 1)
+    ys = np.linspace(0.0, extent[1], nj + 1)
+    zs = np.linspace(0.0, extent[2], nk + 1)
+    X, Y, Z = np.meshgrid(xs, ys, zs, indexing="ij")
+    P = np.stack([X, Y, Z], -1)
+    h = np.array([extent[0] / ni, extent[1] / nj, extent[2] / nk])
+    interior = np.zeros(X.shape, dtype=bool)
+    interior[1:-1, 1:-1, 1:-1] = True
+    P = P + np.where(interior[..., None], rng.uniform(-jitter, jitter, P.shape) * h, 0.0)
+    nodes = P.reshape(-1, 3)
+
+    def node_id(i, j, k):
+        return (i * (nj + 1) + j) * (nk + 1) + k
+
+    tets = _orient_tets(nodes, kuhn_tets(ni, nj, nk, node_id))
+    bf = boundary_faces_of(tets, 3)
+    c = nodes[bf].mean(axis=1)
+    eps = 1e-9
+    names = {
+        "xmin": np.abs(c[:, 0]) < eps, "xmax": np.abs(c[:, 0] - extent[0]) < eps,
+        "ymin": np.abs(c[:, 1]) < eps, "ymax": np.abs(c[:, 1] - extent[1]) < eps,
+        "zmin": np.abs(c[:, 2]) < eps, "zmax": np.abs(c[:, 2] - extent[2]) < eps,
+    }
+    btype = np.full(bf.shape[0], FREESTREAM, dtype=np.int32)
+    for nm in slip_faces:
+        btype[names[nm]] = SLIP_WALL
+    # shuffle face vertex order so the reader has to orient normals itself
+    rot = rng.integers(0, 3, size=bf.shape[0])
+    bf = np.stack([np.roll(f, r) for f, r in zip(bf, rot)]) if bf.shape[0] else bf
+    return MeshData(3, nodes, tets.astype(np.int64), bf.astype(np.int64), btype)
+
+
+def tri_patch(ni=4, nj=3, seed=3, jitter=0.2, slip=("bottom",)):
+    """Small 2D jittered triangle mesh on the unit square (tests)."""
+    rng = np.random.default_rng(seed)
+    xs = np.linspace(0.0, 1.0, ni + 1)
+    ys = np.linspace(0.0, 1.0, nj + 1)
+    X, Y = np.meshgrid(xs, ys, indexing="ij")
+    interior = np.zeros(X.shape, dtype=bool)
+    interior[1:-1, 1:-1] = True
+    X = X + np.where(interior, rng.uniform(-jitter, jitter, X.shape) / ni, 0.0)
+    Y = Y + np.where(interior, rng.uniform(-jitter, jitter, Y.shape) / nj, 0.0)
+    nodes = np.stack([X.ravel(), Y.ravel()], 1)
+
+    def nid(i, j):
+        return i * (nj + 1) + j
+
+    I, J = np.meshgrid(np.arange(ni), np.arange(nj), indexing="ij")
+    I = I.ravel(); J = J.ravel()
+    a, b, c, d = nid(I, J), nid(I + 1, J), nid(I + 1, J + 1), nid(I, J + 1)
+    diag = rng.random(I.shape[0]) < 0.5
+    t1 = np.where(diag[:, None], np.stack([a, b, c], 1), np.stack([a, b, d], 1))
+    t2 = np.where(diag[:, None], np.stack([a, c, d], 1), np.stack([b, c, d], 1))
+    tris = _orient_tris(nodes, np.stack([t1, t2], 1).reshape(-1, 3))
+    bf = boundary_faces_of(tris, 2)
+    cen = nodes[bf].mean(axis=1)
+    eps = 1e-9
+    names = {"bottom": np.abs(cen[:, 1]) < eps, "top": np.abs(cen[:, 1] - 1) < eps,
+             "left": np.abs(cen[:, 0]) < eps, "right": np.abs(cen[:, 0] - 1) < eps}
+    btype = np.full(bf.shape[0], FREESTREAM, dtype=np.int32)
+    for nm in slip:
+        btype[names[nm]] = SLIP_WALL
+    flip = rng.random(bf.shape[0]) < 0.5
+    bf = np.where(flip[:, None], bf[:, ::-1], bf)
+    return MeshData(2, nodes, tris.astype(np.int64), bf.astype(np.int64), btype)
+
+
+# --------------------------------------------------------------------------
+# MSH2 writer / reader (the reference's accepted subset)
+# --------------------------------------------------------------------------
+
+def write_msh2(path, mesh: MeshData):
+    """Write ``mesh`` as Gmsh MSH2 ASCII in the subset the reference reads."""
+    nd = mesh.ndim
+    face_type = 1 if nd == 2 else 2
+    cell_type = 2 if nd == 2 else 4
+    lines = ["$MeshFormat", "2.2 0 8", "$EndMeshFormat", "$PhysicalNames", "3",
+             f"{nd - 1} {SLIP_WALL} \"SLIP_WALL\"",
+             f"{nd - 1} {FREESTREAM} \"FREESTREAM\"",
+             f"{nd} 10 \"FLUID\"", "$EndPhysicalNames", "$Nodes", str(mesh.nnode)]
+    pts = mesh.nodes
+    if nd == 2:
+        pts = np.concatenate([pts, np.zeros((pts.shape[0], 1))], axis=1)
+    ids = np.arange(1, mesh.nnode + 1)
+    lines += [f"{i} {x!r} {y!r} {z!r}" for i, (x, y, z) in zip(ids, pts.tolist())]
+    lines += ["$EndNodes", "$Elements", str(mesh.bfaces.shape[0] + mesh.ncell)]
+    eid = 1
+    for f, t in zip((mesh.bfaces + 1).tolist(), mesh.btype.tolist()):
+        lines.append(f"{eid} {face_type} 2 {t} {t} " + " ".join(map(str, f)))
+        eid += 1
+    for c in (mesh.cells + 1).tolist():
+        lines.append(f"{eid} {cell_type} 2 10 10 " + " ".join(map(str, c)))
+        eid += 1
+    lines += ["$EndElements"]
+    with open(path, "w") as fh:
+        fh.write("\n".join(lines) + "\n")
+
+
+def read_msh2(path) -> MeshData:
+    """Read the MSH2 subset written by :func:`write_msh2`."""
+    with open(path) as fh:
+        text = fh.read().split("\n")
+    it = iter(text)
+    names = {}
+    ndim = 0
+    nodes = None
+    bf, bt, cells = [], [], []
+    for line in it:
+        if line == "$PhysicalNames":
+            n = int(next(it))
+            for _ in range(n):
+                d, tag, nm = next(it).split(" ")
+                ndim = max(ndim, int(d))
+                names[int(tag)] = nm.strip('"')
+        elif line == "$Nodes":
+            n = int(next(it))
+            nodes = np.array([[float(v) for v in next(it).split(" ")[1:1 + 3]] for _ in range(n)])
+        elif line == "$Elements":
+            n = int(next(it))
+            for _ in range(n):
+                v = [int(x) for x in next(it).split(" ")]
+                et, ntag = v[1], v[2]
+                conn = [x - 1 for x in v[3 + ntag:]]
+                if (ndim == 2 and et == 1) or (ndim == 3 and et == 2):
+                    bf.append(conn)
+                    nm = names.get(v[3], "")
+                    bt.append(SLIP_WALL if nm == "SLIP_WALL" else FREESTREAM)
+                elif (ndim == 2 and et == 2) or (ndim == 3 and et == 4):
+                    cells.append(conn)
+    return MeshData(ndim, nodes[:, :ndim].copy(), np.array(cells, dtype=np.int64),
+                    np.array(bf, dtype=np.int64), np.array(bt, dtype=np.int32))
+
+
+def write_besi(path, mesh_file, ndim, mach=(0.5, 0.0, 0.0), solver="EXPLICIT",
+               cfl=0.8, cfl_max=0.0, exponents=(1.0, 1.0), local_dt=True,
+               nsteps=1, output="output"):
+    """Write a reference-format ``.besi`` configuration (Config.cpp:116-422)."""
+    m = ",".join(repr(float(x)) for x in mach[:ndim])
+    lines = [
+        "MESH_FORMAT = GMSH", f"MESH_FILE = {mesh_file}", "OUTPUT_FORMAT = VTK",
+        f"OUTPUT_FOLDER = {output}", "EQUATION = EULER_NON_DIMENSIONAL",
+        f"SOLVER_TYPE = {solver}", "EXPLICIT_SCHEME = MULTI_STEP",
+        "MULTI_TIME_STEP_OPTIONS = 4, 0.0833, 0.2069, 0.4265, 1.0",
+        f"COURANT_NUM = {cfl!r}", f"ADAPTIVE_COURANT_NUM_MAX = {cfl_max!r}",
+        f"ADAPTIVE_COURANT_NUM_EXPONENT = {exponents[0]!r}, {exponents[1]!r}",
+        f"LOCAL_TIME_STEP = {'YES' if local_dt else 'NO'}",
+        f"NUMBER_OF_TIME_STEP = {nsteps}", "SAVE_INTERVAL = 1000000",
+        "FREESTREAM_RAMP = 0", "FREESTREAM_RAMP_START_FRAC = 0.5",
+        "GRADIENT_SCHEME = GREEN_GAUSS", "LIMITER_SCHEME = VENKATAKRISHNAN",
+        "VENKATAKRISHNAN_FACTOR = 5.0", "FLUX_SCHEME = ROE",
+        "HARTEN_CORRECTION_LIMIT = 0.05", "GAS_CONSTANT = 287.0",
+        "SPECIFIC_HEAT_RATIO = 1.4", f"FREESTREAM_MACH = {m}",
+        "AMG_PRECOND_COARSENER = AGGREGATION", "AMG_PRECOND_SMOOTHER = DAMPED_JACOBI",
+        "LINEAR_SOLVER = GMRES", "LINEAR_SOLVER_MAX_ITER = 200",
+        "LINEAR_SOLVER_ABS_TOL = 1E-6",
+    ]
+    with open(path, "w") as fh:
+        fh.write("\n".join(lines) + "\n")
