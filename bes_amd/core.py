@@ -1,0 +1,259 @@
+"""Python binding of the BES solver core C ABI (include/bes_hip.h).
+
+Loads the in-tree ``bes_amd/libbes_hip.so`` and raises if it is missing:
+there is no CPU fallback.  Arrays cross the boundary as plain host buffers in
+the reference's conventions (cell-major ``[ncell][nv]`` states in the mesh
+file's element order).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libbes_hip.so")
+
+OK, ERR_ARG, ERR_HIP, ERR_NAN, ERR_MESH, ERR_SOLVE = range(6)
+INTERNAL, SLIP_WALL, FREESTREAM = 1, 2, 3
+GEOM_COMPAT, GEOM_EXACT = 0, 7
+RK4_ALPHA = (0.0833, 0.2069, 0.4265, 1.0)
+
+
+class BesError(RuntimeError):
+    def __init__(self, status, message, face=-1):
+        super().__init__(f"{message} (status {status})")
+        self.status = status
+        self.face = face
+
+
+class Gas(C.Structure):
+    _fields_ = [("gamma", C.c_double), ("R", C.c_double), ("harten", C.c_double),
+                ("venkat_k", C.c_double)]
+
+
+class MeshDesc(C.Structure):
+    _fields_ = [("ndim", C.c_int), ("nnode", C.c_long), ("coords", C.c_void_p),
+                ("ncell", C.c_long), ("cells", C.c_void_p), ("nbface", C.c_long),
+                ("bfaces", C.c_void_p), ("btype", C.c_void_p)]
+
+
+def default_gas(gamma=1.4, R=287.0, harten=0.05, venkat_k=5.0):
+    return Gas(gamma, R, harten, venkat_k)
+
+
+_lib = None
+
+_SIGS = {
+    "bes_ctx_create": [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p],
+    "bes_ctx_create_msh2": [C.c_char_p, C.c_int, C.c_void_p, C.c_void_p],
+    "bes_ctx_destroy": [C.c_void_p],
+    "bes_ctx_info": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],
+    "bes_get_geometry": [C.c_void_p] * 13,
+    "bes_set_freestream": [C.c_void_p, C.c_void_p],
+    "bes_set_state": [C.c_void_p, C.c_void_p],
+    "bes_get_state": [C.c_void_p, C.c_void_p],
+    "bes_compute_time_step": [C.c_void_p, C.c_double, C.c_int],
+    "bes_get_time_step": [C.c_void_p, C.c_void_p],
+    "bes_compute_gradient_limiter": [C.c_void_p],
+    "bes_get_gradient": [C.c_void_p, C.c_void_p],
+    "bes_get_limiter": [C.c_void_p, C.c_void_p],
+    "bes_compute_residual": [C.c_void_p, C.c_void_p, C.c_void_p],
+    "bes_explicit_run": [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_double, C.c_int, C.c_void_p],
+    "bes_flux_batch": [C.c_int, C.c_int, C.c_long, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],
+    "bes_jacobian_batch": [C.c_int, C.c_int, C.c_long, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                           C.c_void_p, C.c_void_p],
+}
+OPTIONAL_SIGS = {
+    "bes_set_profiling": [C.c_void_p, C.c_int],
+    "bes_get_profile": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int],
+    "bes_synchronize": [C.c_void_p],
+}
+PROF_KINDS = ["timestep", "gradlim", "residual", "jacobian", "assemble", "spmv", "precond",
+              "krylov", "factor", "update"]
+
+
+def lib():
+    """The loaded C ABI; raises if the HIP library has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m bes_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        for name, args in {**_SIGS, **OPTIONAL_SIGS}.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = C.c_int
+        L.bes_last_error_face.argtypes = [C.c_void_p]
+        L.bes_last_error_face.restype = C.c_long
+        L.bes_last_error_message.restype = C.c_char_p
+        L.bes_status_string.restype = C.c_char_p
+        L.bes_status_string.argtypes = [C.c_int]
+        _lib = L
+    return _lib
+
+
+def exported_symbols():
+    """Names declared in include/bes_hip.h (for the export check)."""
+    import re
+    with open(os.path.join(os.path.dirname(HERE), "include", "bes_hip.h")) as fh:
+        text = fh.read()
+    return sorted(set(re.findall(r"^\s*(?:int|long|const char \*|void)\s*\*?\s*(bes_\w+)\s*\(", text, re.M)))
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _check(status, ctx=None):
+    if status != OK:
+        L = lib()
+        msg = L.bes_last_error_message().decode(errors="replace")
+        face = L.bes_last_error_face(ctx) if ctx else -1
+        raise BesError(status, msg or L.bes_status_string(status).decode(), face)
+
+
+class Context:
+    """A mesh, its geometry and its flow state resident on the GPU."""
+
+    def __init__(self, mesh=None, msh2_path=None, geom_mode=GEOM_COMPAT, gas=None):
+        L = lib()
+        self.gas = gas or default_gas()
+        h = C.c_void_p()
+        if msh2_path is not None:
+            _check(L.bes_ctx_create_msh2(msh2_path.encode(), geom_mode, C.byref(self.gas), C.byref(h)))
+        else:
+            self._arrays = [np.ascontiguousarray(mesh.nodes, np.float64),
+                            np.ascontiguousarray(mesh.cells, np.int64),
+                            np.ascontiguousarray(mesh.bfaces, np.int64),
+                            np.ascontiguousarray(mesh.btype, np.int32)]
+            X, cells, bf, bt = self._arrays
+            desc = MeshDesc(mesh.ndim, X.shape[0], _p(X), cells.shape[0], _p(cells), bf.shape[0],
+                            _p(bf), _p(bt))
+            _check(L.bes_ctx_create(C.byref(desc), geom_mode, C.byref(self.gas), C.byref(h)))
+            self._arrays = None
+        self.h = h
+        nd, nc, nf = C.c_int(), C.c_long(), C.c_long()
+        _check(L.bes_ctx_info(h, C.byref(nd), C.byref(nc), C.byref(nf)))
+        self.ndim, self.ncell, self.nface = nd.value, nc.value, nf.value
+        self.nv = self.ndim + 2
+        self.k = self.ndim + 1
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().bes_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ---- geometry ----
+    def geometry(self):
+        nf, ne, nd, k = self.nface, self.ncell, self.ndim, self.k
+        o = dict(owner=np.zeros(nf, np.int64), nbr=np.zeros(nf, np.int64), btype=np.zeros(nf, np.int32),
+                 S=np.zeros((nf, nd)), n=np.zeros((nf, nd)), area=np.zeros(nf), vol=np.zeros(ne),
+                 centroid=np.zeros((ne, nd)), cface=np.zeros((ne, k), np.int64),
+                 cadj=np.zeros((ne, k), np.int64), w=np.zeros((ne, k)), r=np.zeros((ne, k, nd)))
+        _check(lib().bes_get_geometry(self.h, *[_p(o[x]) for x in
+                                                ("owner", "nbr", "btype", "S", "n", "area", "vol", "centroid",
+                                                 "cface", "cadj", "w", "r")]))
+        return o
+
+    # ---- state ----
+    def set_freestream(self, primitive):
+        p = np.ascontiguousarray(primitive, np.float64)
+        assert p.shape == (self.nv + 1,)
+        _check(lib().bes_set_freestream(self.h, _p(p)))
+
+    def set_state(self, U):
+        U = np.ascontiguousarray(U, np.float64)
+        assert U.shape == (self.ncell, self.nv)
+        _check(lib().bes_set_state(self.h, _p(U)))
+
+    def get_state(self):
+        U = np.zeros((self.ncell, self.nv))
+        _check(lib().bes_get_state(self.h, _p(U)))
+        return U
+
+    # ---- passes ----
+    def time_step(self, cfl, local=True):
+        _check(lib().bes_compute_time_step(self.h, cfl, int(local)))
+        dt = np.zeros(self.ncell)
+        _check(lib().bes_get_time_step(self.h, _p(dt)))
+        return dt
+
+    def gradient_limiter(self):
+        _check(lib().bes_compute_gradient_limiter(self.h))
+        G = np.zeros((self.ncell, self.nv, self.ndim))
+        P = np.zeros((self.ncell, self.nv))
+        _check(lib().bes_get_gradient(self.h, _p(G)))
+        _check(lib().bes_get_limiter(self.h, _p(P)))
+        return G, P
+
+    def residual(self):
+        """Second-order residual of the current state (gradient/limiter recomputed)."""
+        _check(lib().bes_compute_gradient_limiter(self.h))
+        R = np.zeros((self.ncell, self.nv))
+        mx = np.zeros(self.nv)
+        _check(lib().bes_compute_residual(self.h, _p(R), _p(mx)), self.h)
+        return R, mx
+
+    def set_profiling(self, on=True):
+        _check(lib().bes_set_profiling(self.h, int(on)))
+
+    def profile(self):
+        n = len(PROF_KINDS)
+        ms = np.zeros(n)
+        cnt = np.zeros(n, np.int64)
+        _check(lib().bes_get_profile(self.h, _p(ms), _p(cnt), n))
+        return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(PROF_KINDS) if cnt[i] > 0}
+
+    def synchronize(self):
+        _check(lib().bes_synchronize(self.h))
+
+    def explicit_run(self, niter, cfl, local=True, alpha=RK4_ALPHA, monitor=True):
+        a = np.ascontiguousarray(alpha, np.float64)
+        linf = np.zeros((niter, self.nv)) if monitor else None
+        _check(lib().bes_explicit_run(self.h, niter, len(a), _p(a), cfl, int(local), _p(linf)), self.h)
+        return linf
+
+
+def flux_batch(ndim, kind, UL, UR, normals, gas=None):
+    gas = gas or default_gas()
+    nv = ndim + 2
+    UL = np.ascontiguousarray(UL, np.float64).reshape(-1, nv)
+    n = UL.shape[0]
+    URp = np.zeros((n, nv + 1))
+    if UR is not None:
+        UR = np.asarray(UR, np.float64).reshape(n, -1)
+        URp[:, :UR.shape[1]] = UR
+    nrm = np.ascontiguousarray(normals, np.float64).reshape(n, ndim)
+    F = np.zeros((n, nv))
+    _check(lib().bes_flux_batch(ndim, kind, n, _p(UL), _p(URp), _p(nrm), C.byref(gas), _p(F)))
+    return F
+
+
+def jacobian_batch(ndim, kind, UL, UR, normals, gas=None):
+    gas = gas or default_gas()
+    nv = ndim + 2
+    UL = np.ascontiguousarray(UL, np.float64).reshape(-1, nv)
+    n = UL.shape[0]
+    URp = np.zeros((n, nv + 1))
+    if UR is not None:
+        UR = np.asarray(UR, np.float64).reshape(n, -1)
+        URp[:, :UR.shape[1]] = UR
+    nrm = np.ascontiguousarray(normals, np.float64).reshape(n, ndim)
+    JL = np.zeros((n, nv, nv))
+    JR = np.zeros((n, nv, nv))
+    _check(lib().bes_jacobian_batch(ndim, kind, n, _p(UL), _p(URp), _p(nrm), C.byref(gas), _p(JL), _p(JR)))
+    return JL, JR

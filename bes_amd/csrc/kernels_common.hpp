@@ -1,0 +1,64 @@
+// Launch-shape helpers shared by the BES kernels (gfx950: wave64, 8 XCDs).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace bes {
+
+// Workgroups are dispatched round-robin over the 8 XCDs of an MI355X, each
+// with its own L2.  Remap the hardware block id so that XCD x processes one
+// contiguous range of blocks: Morton-ordered neighbours then share an L2.
+// The map is a bijection on [0, gridDim.x) for any grid size.
+__device__ __forceinline__ int xcd_block() {
+  const int nb = gridDim.x, b = blockIdx.x;
+  const int full = nb >> 3, rem = nb & 7, x = b & 7, q = b >> 3;
+  const int start = x * full + (x < rem ? x : rem);
+  return start + q;
+}
+
+__device__ __forceinline__ int cell_index() { return xcd_block() * blockDim.x + threadIdx.x; }
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ __forceinline__ double wave_min(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Block reductions (blockDim.x a multiple of 64, at most 1024).  Every thread
+// of the block must call them; the result is valid in thread 0.
+__device__ __forceinline__ double block_max(double v) {
+  __shared__ double s[16];
+  v = wave_max(v);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  __syncthreads();
+  if (l == 0) s[w] = v;
+  __syncthreads();
+  double r = -__builtin_inf();
+  if (threadIdx.x == 0)
+    for (int i = 0; i < (int)(blockDim.x >> 6); i++) r = fmax(r, s[i]);
+  return r;
+}
+__device__ __forceinline__ double block_min(double v) {
+  __shared__ double s[16];
+  v = wave_min(v);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  __syncthreads();
+  if (l == 0) s[w] = v;
+  __syncthreads();
+  double r = __builtin_inf();
+  if (threadIdx.x == 0)
+    for (int i = 0; i < (int)(blockDim.x >> 6); i++) r = fmin(r, s[i]);
+  return r;
+}
+
+}  // namespace bes

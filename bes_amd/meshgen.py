@@ -81,8 +81,14 @@ def boundary_faces_of(cells: np.ndarray, ndim: int) -> np.ndarray:
     else:
         loc = [(0, 2, 1), (1, 2, 3), (0, 3, 2), (0, 1, 3)]
     faces = np.concatenate([cells[:, list(l)] for l in loc], axis=0)
-    key = np.sort(faces, axis=1)
-    _, inv, cnt = np.unique(key, axis=0, return_inverse=True, return_counts=True)
+    key = np.sort(faces, axis=1).astype(np.int64)
+    if int(cells.max()) < (1 << 21):
+        packed = key[:, 0]
+        for q in range(1, key.shape[1]):
+            packed = (packed << 21) | key[:, q]
+        _, inv, cnt = np.unique(packed, return_inverse=True, return_counts=True)
+    else:
+        _, inv, cnt = np.unique(key, axis=0, return_inverse=True, return_counts=True)
     inv = inv.reshape(-1)
     once = cnt[inv] == 1
     return faces[once]
@@ -219,7 +225,7 @@ def box_tets(ni=4, nj=3, nk=3, seed=7, jitter=0.15, extent=(1.0, 1.0, 1.0),
         btype[names[nm]] = SLIP_WALL
     # shuffle face vertex order so the reader has to orient normals itself
     rot = rng.integers(0, 3, size=bf.shape[0])
-    bf = np.stack([np.roll(f, r) for f, r in zip(bf, rot)]) if bf.shape[0] else bf
+    bf = np.take_along_axis(bf, (np.arange(3)[None, :] - rot[:, None]) % 3, axis=1)
     return MeshData(3, nodes, tets.astype(np.int64), bf.astype(np.int64), btype)
 
 

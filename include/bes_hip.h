@@ -1,0 +1,141 @@
+/* bes_hip.h — C ABI of the MI355X-native BES solver core (libbes_hip.so).
+ *
+ * The reference (joel-ho/BES) has no FFI: its residual and implicit-update
+ * path is a set of abstract C++ classes called in-process.  Each entry point
+ * below replaces one of those interfaces (cited as reference file:line); the
+ * C++ host classes in include/bes/ and the Python bindings in bes_amd/ sit on
+ * top of this ABI.  Plain pointers and sizes only; every function returns a
+ * bes_status; nothing ever exits the process (the reference calls exit() on
+ * errors, e.g. SolverExplicit.cpp:311-323).
+ *
+ * Conventions (reference Config.cpp:469-477, Face.hpp:33):
+ *   conserved state  [rho, rho*u_d..., rho*E], nv = ndim + 2
+ *   primitive state  [rho, u_d..., p, T],      nv + 1 entries
+ *   host state arrays are cell-major [ncell][nv] in the reference's element
+ *   order (the order of the mesh file); the device keeps its own SoA layout.
+ *   face normals are unit, owner -> neighbour, outward on boundaries.
+ */
+#ifndef BES_HIP_H
+#define BES_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  BES_OK = 0,
+  BES_ERR_ARG = 1,     /* invalid argument / size / mode */
+  BES_ERR_HIP = 2,     /* HIP runtime error (message in bes_last_error_message) */
+  BES_ERR_NAN = 3,     /* non-finite flux; face index via bes_last_error_face */
+  BES_ERR_MESH = 4,    /* mesh file unreadable or outside the supported subset */
+  BES_ERR_SOLVE = 5    /* linear solver failed to reach its tolerance */
+} bes_status;
+
+/* boundary type codes, reference Options.hpp:34-35 */
+enum { BES_INTERNAL = 1, BES_SLIP_WALL = 2, BES_FREESTREAM = 3 };
+
+/* geometry mode bits (0 = the reference's metrics exactly):
+ *   1: true tetrahedron volume  (reference MeshGmsh.cpp:101-133 omits the 1/3)
+ *   2: true tetrahedron centroid (reference MeshGmsh.cpp:135-145 counts v2 twice)
+ *   4: per-face reconstruction vector (reference Element.cpp:102-105 keys its face
+ *      map before face ids exist, so Solver.cpp:91 reads the vector of local face 0) */
+enum { BES_GEOM_COMPAT = 0, BES_GEOM_EXACT = 7 };
+
+typedef struct {
+  double gamma;      /* SPECIFIC_HEAT_RATIO, default 1.4     (Config.cpp:110) */
+  double R;          /* GAS_CONSTANT, default 287             (Config.cpp:109) */
+  double harten;     /* HARTEN_CORRECTION_LIMIT, default 0.05 (Config.cpp:107) */
+  double venkat_k;   /* VENKATAKRISHNAN_FACTOR, default 5     (Config.cpp:106) */
+} bes_gas;
+
+/* Mesh in memory: the element lists of an MSH2 file (Mesh.cpp:95-558).
+ * bfaces: boundary faces (lines in 2D, triangles in 3D) in file order, before
+ * all cells; cells: triangles (CCW) or tetrahedra (positive), file order. */
+typedef struct {
+  int ndim;
+  long nnode;
+  const double *coords;   /* [nnode][ndim] */
+  long ncell;
+  const long *cells;      /* [ncell][ndim+1], 0-based */
+  long nbface;
+  const long *bfaces;     /* [nbface][ndim], 0-based */
+  const int *btype;       /* [nbface] BES_SLIP_WALL / BES_FREESTREAM */
+} bes_mesh_desc;
+
+typedef struct bes_ctx bes_ctx;
+
+/* ---- context: replaces Mesh::initialize + Solution/SolutionSnapshot storage
+ *      (Mesh.cpp:66-93, Solution.cpp:43-61, SolutionSnapshot.cpp:60-86) ---- */
+int bes_ctx_create(const bes_mesh_desc *mesh, int geom_mode, const bes_gas *gas, bes_ctx **out);
+/* reads the reference's MSH2 subset (Mesh.cpp:95-558) */
+int bes_ctx_create_msh2(const char *path, int geom_mode, const bes_gas *gas, bes_ctx **out);
+int bes_ctx_destroy(bes_ctx *ctx);
+int bes_ctx_info(const bes_ctx *ctx, int *ndim, long *ncell, long *nface);
+/* reference-order geometry as the reference computes it (for parity and the
+ * per-element getters); any pointer may be NULL.  Face arrays [nface][...],
+ * cell arrays [ncell][...], k = ndim+1 faces per cell. */
+int bes_get_geometry(const bes_ctx *ctx, long *owner, long *nbr, int *btype, double *S,
+                     double *n, double *area, double *vol, double *centroid,
+                     long *cell_faces, long *cell_adj, double *w, double *r);
+
+/* freestream primitive [nv+1] used by far-field faces (Config.cpp:479-488,688-706) */
+int bes_set_freestream(bes_ctx *ctx, const double *primitive);
+/* SolutionSnapshot::setConservedVariable / getConservedVariable for all cells */
+int bes_set_state(bes_ctx *ctx, const double *U);
+int bes_get_state(bes_ctx *ctx, double *U);
+
+/* ---- TimeStepCalculator::computeAllTimeStep (TimeStepCalculator.hpp:43-46;
+ *      TimeStepCalculatorLocal.cpp:67-93, TimeStepCalculatorGlobal.cpp:64-98) ---- */
+int bes_compute_time_step(bes_ctx *ctx, double cfl, int local);
+int bes_get_time_step(bes_ctx *ctx, double *dt /*[ncell]*/);
+
+/* ---- Gradient::computeGradient + Limiter::computeLimiterTerm (Gradient.hpp:44-49,
+ *      Limiter.hpp:45-51; GradientGreenGauss.cpp:21-135,
+ *      LimiterVenkatakrishnan.cpp:21-128): one fused pass over the current state ---- */
+int bes_compute_gradient_limiter(bes_ctx *ctx);
+int bes_get_gradient(bes_ctx *ctx, double *G /*[ncell][nv][ndim]*/);
+int bes_get_limiter(bes_ctx *ctx, double *phi /*[ncell][nv]*/);
+
+/* ---- second-order residual sum_f +-F*A per cell (the face loop of
+ *      SolverExplicit.cpp:285-344 with Solver.cpp:69-121); uses the gradient and
+ *      limiter of the last bes_compute_gradient_limiter call.  R_out [ncell][nv]
+ *      may be NULL; maxabs_out [nv] (Linf per variable) may be NULL ---- */
+int bes_compute_residual(bes_ctx *ctx, double *R_out, double *maxabs_out);
+
+/* ---- SolverExplicit::timeStep_ (SolverExplicit.cpp:259-393): niter full
+ *      multi-stage pseudo-time steps on the device state.  linf_out [niter][nv]
+ *      (last-stage Linf of the residual, the reference monitor) may be NULL. ---- */
+int bes_explicit_run(bes_ctx *ctx, int niter, int nstage, const double *alpha, double cfl,
+                     int local, double *linf_out);
+
+/* ---- Flux_::computeFlux / FluxJacobian::computeJacobian over a batch of faces
+ *      (Flux.hpp:43-46, FluxJacobian.hpp:36-44; FluxRoe.tpp:306-333,
+ *      FluxFreestream.tpp:192-226, FluxSlipWall.tpp:32-53, FluxJacobianWrapper.tpp:32-82).
+ *      kind = BES_INTERNAL (Roe), BES_SLIP_WALL, BES_FREESTREAM.  UR rows have
+ *      nv+1 entries: the neighbour conserved state (Roe) or the freestream
+ *      primitive state (far field); unused otherwise.  Host arrays in and out. ---- */
+int bes_flux_batch(int ndim, int kind, long n, const double *UL, const double *UR,
+                   const double *normals, const bes_gas *gas, double *F);
+int bes_jacobian_batch(int ndim, int kind, long n, const double *UL, const double *UR,
+                       const double *normals, const bes_gas *gas, double *JL, double *JR);
+
+/* ---- measurement (no counterpart in the reference, which times nothing):
+ *      when profiling is on, every kernel launch is bracketed by HIP events on
+ *      the context's stream; bes_get_profile returns, per kernel kind, the
+ *      summed device time and the launch count since profiling was switched on.
+ *      kinds: 0 time step, 1 gradient+limiter, 2 residual+update,
+ *      3 Jacobian, 4 assembly, 5 SpMV, 6 preconditioner apply, 7 Krylov vector
+ *      ops, 8 factorisation, 9 implicit update ---- */
+int bes_set_profiling(bes_ctx *ctx, int on);
+int bes_get_profile(bes_ctx *ctx, double *total_ms, long *launches, int nkinds);
+int bes_synchronize(bes_ctx *ctx);
+
+/* ---- error reporting ---- */
+long bes_last_error_face(const bes_ctx *ctx);      /* reference face id of the first NaN flux */
+const char *bes_last_error_message(void);
+const char *bes_status_string(int status);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BES_HIP_H */
