@@ -2,15 +2,28 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
 
-Prints one JSON line.  A "step" is one full pseudo-time step of the solver
-over the whole mesh (all RK stages, or one backward-Euler step including its
-linear solve); value = cells x steps / wall time of the timed region, with
-the mesh and state resident in HBM before timing starts.  Per-kernel device
-times come from HIP events recorded on the solver's own stream during the
-timed region; the roofline entry uses the dominant kernel's algorithmic bytes
-(every array it must touch counted once) over its average launch time.
-The CPU baseline is the oracle (a restatement of the reference loops,
-OpenMP over faces/cells) timed on a bounded sample on this host.
+Prints one JSON line on stdout (progress goes to stderr).
+
+A "step" is one full pseudo-time step over the whole mesh: one backward-Euler
+step including its Jacobian assembly and linear solve (implicit workloads), or
+all RK stages (explicit workloads).  value = cells x K / wall time of exactly K
+timed steps, with mesh and state resident in HBM before timing starts.
+
+Default workload ("implicit-wing-2m"): the 3D transonic implicit case, plain
+FP64 on the GPU -- a swept tapered wing (30 deg LE sweep, taper 0.56, AR 3.8,
+10% closed-TE 4-digit section) in a half-span box with the far field at 10
+chords, ~2M tetrahedra, Mach 0.84, alpha 3.06 deg, backward Euler with the
+first-order Roe Jacobian in 5x5 blocks, ILU(0)-GMRES(30), CFL ramp 10 -> 1000.
+Besides the K timed steps it measures the time to a 1e-8 drop of the density
+residual Linf (sum_f F*A) for the plain FP64 form and for the mixed form
+(FP32 Jacobian storage, preconditioner and Krylov solve under the FP64
+residual), and the lift/drag coefficient difference between the two.
+
+Per-kernel device times come from HIP events recorded on the solver's stream
+during the timed region; the library also sums each launch's algorithmic
+bytes (every array the kernel must touch, counted once), so roofline.achieved
+= algorithmic bytes / device time of the dominant kernel.  The CPU baseline is
+the oracle (a restatement of the reference loops, OpenMP) on a bounded sample.
 """
 from __future__ import annotations
 
@@ -27,20 +40,169 @@ sys.path.insert(0, ROOT)
 
 METRIC = "cell-updates/s + time to 1e-8 residual drop: plain FP64 vs mixed/learned"
 HBM_PEAK_GBS = 8000.0
+KERNEL_NAMES = {"timestep": "k_timestep", "gradlim": "k_gradlim", "residual": "k_residual",
+                "jacobian": "k_face_jacobian", "assemble": "k_assemble", "spmv": "k_spmv",
+                "precond": "k_ilu_lower+k_ilu_upper", "krylov": "k_mdot+k_orth_update",
+                "factor": "k_ilu_factor_color", "update": "k_update"}
 
 
-def alg_bytes(kind, nd):
-    """Algorithmic bytes per cell of one launch (each array counted once)."""
-    nv, k = nd + 2, nd + 1
-    s = 8
-    if kind == "residual":   # k_residual: residual + stage update
-        return nv * s + nv * nd * s + k * (4 + 8 + 3 * nd * s) + nv * s + 2 * s + 4 + nv * s
-    if kind == "gradlim":    # k_gradlim: gradient + limiter, writes phi*grad
-        return nv * s + k * (4 + 2 * nd * s + s) + 2 * s + nv * nd * s
-    if kind == "timestep":
-        return nv * s + nd * s + 2 * s
-    raise KeyError(kind)
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
 
+
+def roofline(prof):
+    kern = {k: v for k, v in prof.items() if v["ms"] > 0}
+    dom = max(kern, key=lambda k: kern[k]["ms"])
+    d = kern[dom]
+    avg_ms = d["ms"] / d["launches"]
+    return {"bound": "hbm", "kernel": KERNEL_NAMES.get(dom, dom), "achieved": round(d["GBps"], 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(d["GBps"] / HBM_PEAK_GBS, 4),
+            "traffic": None, "alg_bytes_per_launch": round(d["bytes"] / d["launches"]),
+            "avg_launch_us": round(avg_ms * 1e3, 2),
+            "kernels": {KERNEL_NAMES.get(k, k): {"total_ms": round(v["ms"], 3), "launches": v["launches"],
+                                                  "avg_us": round(v["ms"] / v["launches"] * 1e3, 2),
+                                                  "GBps": round(v["GBps"], 1)}
+                        for k, v in sorted(kern.items(), key=lambda kv: -kv[1]["ms"])}}
+
+
+# --------------------------------------------------------------------------
+# implicit wing (default)
+# --------------------------------------------------------------------------
+
+def implicit_workload(args):
+    from bes_amd import core, meshgen as mg
+    from oracle import oracle as orc
+
+    t0 = time.perf_counter()
+    md = mg.swept_wing(*args.wing)
+    spec = md.wing["spec"]
+    alpha = 3.06
+    mach = 0.84
+    mvec = (mach * np.cos(np.radians(alpha)), 0.0, mach * np.sin(np.radians(alpha)))
+    finf = orc.freestream_primitive(mvec, 3)
+    U0 = np.tile(orc.conserved_from_primitive(finf, 3), (md.ncell, 1))
+    log(f"mesh {md.ncell} tets in {time.perf_counter() - t0:.1f}s")
+    cfl = (10.0, 1000.0, (1.1, 0.75))
+    out = {}
+    with core.Context(md) as c:
+        c.set_freestream(finf)
+        geo = c.geometry()
+        log(f"context ready at {time.perf_counter() - t0:.1f}s")
+
+        def forces():
+            U = c.get_state()
+            return mg.force_coefficients(geo["S"], _centers(md, geo), geo["btype"], geo["owner"], U, mach,
+                                         alpha, spec)[:2]
+
+        def drop_run(precision, max_steps):
+            c.implicit_setup(precision, "ilu0", restart=30, maxiter=args.krylov_max, rtol=args.rtol)
+            c.set_state(U0)
+            c.synchronize()
+            t = time.perf_counter()
+            hist = c.implicit_run(max_steps, cfl[0], cfl[1], cfl[2], stop_drop=1e-8)
+            c.synchronize()
+            wall = time.perf_counter() - t
+            r = hist[:, 5]
+            drop = float(r[-1] / r[0])
+            cl, cd = forces()
+            log(f"  {precision}-bit: {len(hist)} steps, drop {drop:.2e}, {wall:.2f}s, "
+                f"krylov {int(hist[:, 11].sum())}, CL {cl:.8f} CD {cd:.8f}")
+            return dict(steps=len(hist), wall_s=wall, drop=drop, reached=bool(drop <= 1e-8),
+                        krylov_total=int(hist[:, 11].sum()), CL=cl, CD=cd,
+                        history_density_linf=[float(x) for x in r[::max(1, len(r) // 50)]])
+
+        # K timed steps of the plain FP64 solve (after W warm-up steps)
+        c.implicit_setup(64, "ilu0", restart=30, maxiter=args.krylov_max, rtol=args.rtol)
+        c.set_state(U0)
+        c.implicit_run(args.warmup, cfl[0], cfl[1], cfl[2])
+        c.set_state(U0)
+        c.synchronize()
+        c.set_profiling(True)
+        t = time.perf_counter()
+        hist = c.implicit_run(args.steps, cfl[0], cfl[1], cfl[2])
+        c.synchronize()
+        wall = time.perf_counter() - t
+        prof = c.profile()
+        c.set_profiling(False)
+        assert len(hist) == args.steps
+        log(f"timed {args.steps} steps in {wall:.3f}s")
+        value = md.ncell * args.steps / wall
+        out["roofline"] = roofline(prof)
+        if not args.no_drop:
+            plain = drop_run(64, args.max_steps)
+            mixed = drop_run(32, args.max_steps)
+            out["time_to_drop"] = {
+                "target_drop": 1e-8, "monitor": "Linf over cells of the density residual sum_f F*A",
+                "plain_fp64": plain, "mixed_fp32_inner": mixed,
+                "speedup_mixed_vs_plain": plain["wall_s"] / mixed["wall_s"],
+                "dCL_mixed_vs_plain": abs(mixed["CL"] - plain["CL"]),
+                "dCD_mixed_vs_plain": abs(mixed["CD"] - plain["CD"]),
+                "learned_preconditioner": "not built yet"}
+    cfgd = {"workload": "implicit-wing-2m" if args.wing == [96, 48, 72] else "implicit-wing",
+            "mesh": f"swept tapered wing (LE sweep 30 deg, taper 0.56, AR 3.8, 10% closed-TE 4-digit "
+                    f"section), half-span box, far field 10 chords, {args.wing[0]}x{args.wing[1]}x"
+                    f"{args.wing[2]} lattice x 6 = {md.ncell} tetrahedra",
+            "cells": md.ncell, "mach": mach, "alpha_deg": alpha,
+            "scheme": "backward Euler, 2nd-order Roe residual (Green-Gauss, Venkatakrishnan K=5), "
+                      "first-order Roe Jacobian in 5x5 blocks (nV-fold repeat as the reference), "
+                      "ILU(0)-GMRES(30) rtol %g, CFL 10->1000 exponents (1.1, 0.75), local dt" % args.rtol,
+            "precision": "FP64 throughout"}
+    res = {"metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic", "config": cfgd}
+    res.update(out)
+    if not args.no_cpu:
+        res["cpu_baseline"] = cpu_baseline_implicit(args)
+    return res
+
+
+def _centers(md, geo):
+    """Face centres of the reference face numbering (from the node coordinates)."""
+    # faces: boundary faces keep file order at the front; their centres are what
+    # the wing mask needs, internal faces get their owner centroid (unused).
+    nf = geo["S"].shape[0]
+    c = np.zeros((nf, md.ndim))
+    nb = md.bfaces.shape[0]
+    c[:nb] = md.nodes[md.bfaces].mean(axis=1)
+    c[nb:] = geo["centroid"][geo["owner"][nb:]]
+    return c
+
+
+def cpu_baseline_implicit(args):
+    """The oracle's implicit assembly (reference loops) on a sample, per cell."""
+    from bes_amd import meshgen as mg
+    from oracle import oracle as orc
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    md = mg.swept_wing(24, 12, 18)
+    mvec = (0.84 * np.cos(np.radians(3.06)), 0.0, 0.84 * np.sin(np.radians(3.06)))
+    finf = orc.freestream_primitive(mvec, 3)
+    U = orc.perturbed_state(md.ncell, 3, mvec, seed=1, amp=0.01)
+    m = orc.Mesh(md)
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    nv = 5
+    t0 = time.perf_counter()
+    n = 0
+    while time.perf_counter() - t0 < args.cpu_seconds:
+        rp, col, val, rhs, _ = m.implicit_assemble(U, 10.0, finf)
+        # block CSR system, block-Jacobi preconditioned GMRES(30) as the step's solve
+        A = sp.bsr_matrix((val, col, rp), shape=(md.ncell * nv, md.ncell * nv)).tocsr()
+        Dinv = np.linalg.inv(val[rp[:-1]])
+        M = spla.LinearOperator(A.shape, lambda r: np.einsum("nij,nj->ni", Dinv, r.reshape(-1, nv)).ravel())
+        x, info = spla.gmres(A, rhs.ravel(), M=M, restart=30, maxiter=100, rtol=1e-3)
+        U = U + x.reshape(U.shape)
+        n += 1
+    dt = time.perf_counter() - t0
+    return {"value": md.ncell * n / dt, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+            "sample": f"{n} backward-Euler steps on a {md.ncell}-tet wing mesh, {dt:.1f} s: oracle "
+                      f"assembly (reference face loop, Eigen-rule AD Jacobians, serial) + scipy "
+                      f"block-Jacobi GMRES(30) rtol 1e-3 standing in for AMGCL (not buildable here: "
+                      f"needs Boost); one thread (host threads available: {threads})"}
+
+
+# --------------------------------------------------------------------------
+# explicit workloads
+# --------------------------------------------------------------------------
 
 def explicit_workload(args):
     from bes_amd import core, meshgen as mg
@@ -75,34 +237,12 @@ def explicit_workload(args):
         wall = time.perf_counter() - t0
         prof = c.profile()
         c.set_profiling(False)
-        # same run without events, for the headline number
-        c.set_state(U0)
-        c.explicit_run(args.warmup, 0.8)
-        c.synchronize()
-        t0 = time.perf_counter()
-        c.explicit_run(args.steps, 0.8)
-        c.synchronize()
-        wall_noev = time.perf_counter() - t0
-    cells = md.ncell
-    ms_step = wall_noev * 1e3 / args.steps
-    value = cells * args.steps / wall_noev
-    kern = {}
-    for kname, (tot, cnt) in prof.items():
-        avg = tot / cnt
-        b = alg_bytes(kname, nd) * cells
-        kern[kname] = {"avg_ms": avg, "launches": cnt, "alg_bytes": b, "GBps": b / (avg * 1e-3) / 1e9}
-    dom = max(kern, key=lambda k: kern[k]["avg_ms"] * kern[k]["launches"])
-    roof = {"bound": "hbm", "kernel": "k_" + dom, "achieved": round(kern[dom]["GBps"], 1),
-            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(kern[dom]["GBps"] / HBM_PEAK_GBS, 4),
-            "traffic": None, "alg_bytes_per_cell": alg_bytes(dom, nd),
-            "kernels": {k: {"avg_us": round(v["avg_ms"] * 1e3, 2), "GBps": round(v["GBps"], 1),
-                            "launches": v["launches"]} for k, v in kern.items()}}
-    cfg["cells"] = cells
-    out = {"metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
-           "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
-           "vs_baseline": None, "dtype": "f64", "data": "synthetic", "config": cfg, "roofline": roof,
-           "residual_linf_last": [float(x) for x in linf[-1]],
-           "wall_with_events_s": wall}
+    cfg["cells"] = md.ncell
+    out = {"metric": METRIC, "value": md.ncell * args.steps / wall, "unit": "cell-updates/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic", "config": cfg, "roofline": roofline(prof),
+           "residual_linf_last": [float(x) for x in linf[-1]]}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline_explicit(args, nd)
     return out
@@ -139,19 +279,27 @@ def cpu_baseline_explicit(args, nd):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="explicit-tetbox",
-                    choices=["explicit-tetbox", "explicit-bump-40k"])
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="implicit-wing-2m",
+                    choices=["implicit-wing-2m", "explicit-tetbox", "explicit-bump-40k"])
+    ap.add_argument("--wing", type=int, nargs=3, default=[96, 48, 72])
     ap.add_argument("--box", type=int, nargs=3, default=[80, 64, 65])
+    ap.add_argument("--max-steps", type=int, default=300)
+    ap.add_argument("--krylov-max", type=int, default=100)
+    ap.add_argument("--rtol", type=float, default=1e-3)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-drop", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is designed "
                           "(see DESIGN.md) but not built", "n_gpus": args.gpus}))
         sys.exit(2)
-    out = explicit_workload(args)
+    if args.workload.startswith("implicit"):
+        out = implicit_workload(args)
+    else:
+        out = explicit_workload(args)
     print(json.dumps(out))
 
 

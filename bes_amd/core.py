@@ -66,8 +66,16 @@ _SIGS = {
                            C.c_void_p, C.c_void_p],
 }
 OPTIONAL_SIGS = {
+    "bes_implicit_setup": [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double],
+    "bes_implicit_assemble": [C.c_void_p, C.c_double, C.c_int, C.c_int, C.c_void_p, C.c_void_p],
+    "bes_implicit_get_system": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],
+    "bes_linear_solve": [C.c_void_p, C.c_void_p, C.c_void_p],
+    "bes_get_linear_solution": [C.c_void_p, C.c_void_p],
+    "bes_implicit_update": [C.c_void_p],
+    "bes_implicit_run": [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_void_p, C.c_int, C.c_int,
+                         C.c_double, C.c_void_p, C.c_void_p],
     "bes_set_profiling": [C.c_void_p, C.c_int],
-    "bes_get_profile": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int],
+    "bes_get_profile": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int],
     "bes_synchronize": [C.c_void_p],
 }
 PROF_KINDS = ["timestep", "gradlim", "residual", "jacobian", "assemble", "spmv", "precond",
@@ -85,6 +93,8 @@ def lib():
             fn = getattr(L, name)
             fn.argtypes = args
             fn.restype = C.c_int
+        L.bes_implicit_nnz.argtypes = [C.c_void_p]
+        L.bes_implicit_nnz.restype = C.c_long
         L.bes_last_error_face.argtypes = [C.c_void_p]
         L.bes_last_error_face.restype = C.c_long
         L.bes_last_error_message.restype = C.c_char_p
@@ -215,11 +225,62 @@ class Context:
         n = len(PROF_KINDS)
         ms = np.zeros(n)
         cnt = np.zeros(n, np.int64)
-        _check(lib().bes_get_profile(self.h, _p(ms), _p(cnt), n))
-        return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(PROF_KINDS) if cnt[i] > 0}
+        by = np.zeros(n)
+        _check(lib().bes_get_profile(self.h, _p(ms), _p(cnt), _p(by), n))
+        return {k: dict(ms=float(ms[i]), launches=int(cnt[i]), bytes=float(by[i]),
+                        GBps=float(by[i]) / (ms[i] * 1e-3) / 1e9 if ms[i] > 0 else 0.0)
+                for i, k in enumerate(PROF_KINDS) if cnt[i] > 0}
 
     def synchronize(self):
         _check(lib().bes_synchronize(self.h))
+
+    # ---- implicit path ----
+    PRECOND = {"none": 0, "jacobi": 1, "ilu0": 2}
+
+    def implicit_setup(self, precision=64, precond="ilu0", restart=30, maxiter=1000, rtol=1e-7):
+        _check(lib().bes_implicit_setup(self.h, precision, self.PRECOND[precond], restart, maxiter, rtol))
+
+    def implicit_assemble(self, cfl, local=True, repeat_nv=True):
+        lb = np.zeros(self.nv)
+        lr = np.zeros(self.nv)
+        _check(lib().bes_implicit_assemble(self.h, cfl, int(local), int(repeat_nv), _p(lb), _p(lr)), self.h)
+        return lb, lr
+
+    def implicit_system(self):
+        """Assembled system in the reference CoefficientMatrix CSR layout."""
+        nnz = lib().bes_implicit_nnz(self.h)
+        rp = np.zeros(self.ncell * self.nv + 1, np.int64)
+        col = np.zeros(nnz, np.int64)
+        val = np.zeros(nnz)
+        rhs = np.zeros((self.ncell, self.nv))
+        _check(lib().bes_implicit_get_system(self.h, _p(rp), _p(col), _p(val), _p(rhs)))
+        return rp, col, val, rhs
+
+    def linear_solve(self, allow_unconverged=False):
+        it = C.c_int()
+        rr = C.c_double()
+        st = lib().bes_linear_solve(self.h, C.byref(it), C.byref(rr))
+        if not (allow_unconverged and st == ERR_SOLVE):
+            _check(st, self.h)
+        return it.value, rr.value
+
+    def linear_solution(self):
+        x = np.zeros((self.ncell, self.nv))
+        _check(lib().bes_get_linear_solution(self.h, _p(x)))
+        return x
+
+    def implicit_update(self):
+        _check(lib().bes_implicit_update(self.h))
+
+    def implicit_run(self, nsteps, cfl_min, cfl_max=0.0, exponents=(1.0, 1.0), local=True, stop_drop=0.0,
+                     repeat_nv=True):
+        """Backward-Euler steps; repeat_nv=True is the reference's nV-fold block repeat."""
+        ex = np.ascontiguousarray(exponents, np.float64)
+        hist = np.zeros((nsteps, 2 * self.nv + 3))
+        done = C.c_int()
+        _check(lib().bes_implicit_run(self.h, nsteps, cfl_min, cfl_max, _p(ex), int(local), int(repeat_nv),
+                                      stop_drop, _p(hist), C.byref(done)), self.h)
+        return hist[:done.value]
 
     def explicit_run(self, niter, cfl, local=True, alpha=RK4_ALPHA, monitor=True):
         a = np.ascontiguousarray(alpha, np.float64)

@@ -70,8 +70,10 @@ struct bes_ctx {
   static constexpr int kProfKinds = 16;
   bool prof = false;
   std::vector<hipEvent_t> ev_pool;
-  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_pending;
+  struct PendingEvent { int kind; double bytes; hipEvent_t a, b; };
+  std::vector<PendingEvent> ev_pending;
   double prof_ms[kProfKinds] = {0};
+  double prof_bytes[kProfKinds] = {0};
   long prof_n[kProfKinds] = {0};
 };
 
@@ -89,11 +91,13 @@ enum ProfKind { kProfTimestep = 0, kProfGradlim = 1, kProfResidual = 2, kProfJac
                 kProfAssemble = 4, kProfSpmv = 5, kProfPrecond = 6, kProfKrylov = 7, kProfFactor = 8,
                 kProfUpdate = 9 };
 // bracket a launch with events when profiling is on
+// bytes: the launch's algorithmic HBM bytes (every array it must touch, once)
 struct ProfScope {
   bes_ctx *c;
   int kind;
+  double bytes;
   hipEvent_t a = nullptr, b = nullptr;
-  ProfScope(bes_ctx *c_, int kind_);
+  ProfScope(bes_ctx *c_, int kind_, double bytes_ = 0.0);
   ~ProfScope();
 };
 int prof_collect(bes_ctx *c);

@@ -48,7 +48,7 @@ static hipEvent_t take_event(bes_ctx *c) {
   return e;
 }
 
-ProfScope::ProfScope(bes_ctx *c_, int kind_) : c(c_), kind(kind_) {
+ProfScope::ProfScope(bes_ctx *c_, int kind_, double bytes_) : c(c_), kind(kind_), bytes(bytes_) {
   if (!c->prof) return;
   a = take_event(c);
   b = take_event(c);
@@ -57,7 +57,7 @@ ProfScope::ProfScope(bes_ctx *c_, int kind_) : c(c_), kind(kind_) {
 ProfScope::~ProfScope() {
   if (!c->prof || !a || !b) return;
   hipEventRecord(b, c->stream);
-  c->ev_pending.push_back({kind, {a, b}});
+  c->ev_pending.push_back({kind, bytes, a, b});
 }
 
 int prof_collect(bes_ctx *c) {
@@ -65,11 +65,12 @@ int prof_collect(bes_ctx *c) {
   BES_HIP(hipStreamSynchronize(c->stream));
   for (auto &p : c->ev_pending) {
     float ms = 0;
-    BES_HIP(hipEventElapsedTime(&ms, p.second.first, p.second.second));
-    c->prof_ms[p.first] += ms;
-    c->prof_n[p.first] += 1;
-    c->ev_pool.push_back(p.second.first);
-    c->ev_pool.push_back(p.second.second);
+    BES_HIP(hipEventElapsedTime(&ms, p.a, p.b));
+    c->prof_ms[p.kind] += ms;
+    c->prof_n[p.kind] += 1;
+    c->prof_bytes[p.kind] += p.bytes;
+    c->ev_pool.push_back(p.a);
+    c->ev_pool.push_back(p.b);
   }
   c->ev_pending.clear();
   return BES_OK;
@@ -513,7 +514,7 @@ int launch_timestep(bes_ctx *c, const double *U, double cfl, int local) {
     gm = c->red;
   }
   {
-    ProfScope ps(c, kProfTimestep);
+    ProfScope ps(c, kProfTimestep, (double)c->N * 8 * (c->nv + c->nd + 2));
     if (c->nd == 2) hipLaunchKernelGGL(k_timestep<2>, dim3(nb), dim3(bs), 0, c->stream, U, c->dm, c->gc, cfl, c->dt, gm);
     else hipLaunchKernelGGL(k_timestep<3>, dim3(nb), dim3(bs), 0, c->stream, U, c->dm, c->gc, cfl, c->dt, gm);
   }
@@ -524,7 +525,8 @@ int launch_timestep(bes_ctx *c, const double *U, double cfl, int local) {
 
 int launch_gradlim(bes_ctx *c, const double *U, double *Gout, double *Pout) {
   const int bs = 256, nb = grid_for(c->N, bs);
-  ProfScope ps(c, kProfGradlim);
+  ProfScope ps(c, kProfGradlim,
+               (double)c->N * (8 * c->nv + c->k * (4 + 16 * c->nd + 8) + 16 + 8 * c->nv * c->nd));
   if (c->nd == 2) hipLaunchKernelGGL(k_gradlim<2>, dim3(nb), dim3(bs), 0, c->stream, U, c->dm, c->LG, Gout, Pout);
   else hipLaunchKernelGGL(k_gradlim<3>, dim3(nb), dim3(bs), 0, c->stream, U, c->dm, c->LG, Gout, Pout);
   BES_HIP(hipGetLastError());
@@ -537,7 +539,9 @@ int launch_residual(bes_ctx *c, const double *Us, const double *U0, double *Ud, 
   Finf fi;
   std::memcpy(fi.p, c->finf, sizeof(fi.p));
   const double *dtg = local ? nullptr : c->dtg;
-  ProfScope ps(c, kProfResidual);
+  ProfScope ps(c, kProfResidual,
+               (double)c->N * (8 * c->nv + 8 * c->nv * c->nd + c->k * (12 + 24 * c->nd) + 20 +
+                               (U0 ? 16 * c->nv : 0) + (Rout ? 8 * c->nv : 0)));
   if (c->nd == 2)
     hipLaunchKernelGGL(k_residual<2>, dim3(nb), dim3(bs), 0, c->stream, Us, c->LG, U0, Ud, c->dt, dtg,
                        alpha, c->dm, c->gc, fi, Rout, linf_bits, c->err);
@@ -636,7 +640,7 @@ int bes_ctx_destroy(bes_ctx *c) {
                   c->dt, c->dtg, c->R, c->red, c->err, c->linf};
   for (void *p : ptrs)
     if (p) hipFree(p);
-  for (auto &p : c->ev_pending) { c->ev_pool.push_back(p.second.first); c->ev_pool.push_back(p.second.second); }
+  for (auto &p : c->ev_pending) { c->ev_pool.push_back(p.a); c->ev_pool.push_back(p.b); }
   for (hipEvent_t e : c->ev_pool) if (e) hipEventDestroy(e);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
@@ -844,17 +848,18 @@ int bes_set_profiling(bes_ctx *c, int on) {
   int s = prof_collect(c);
   if (s) return s;
   c->prof = on != 0;
-  for (int q = 0; q < bes_ctx::kProfKinds; q++) { c->prof_ms[q] = 0; c->prof_n[q] = 0; }
+  for (int q = 0; q < bes_ctx::kProfKinds; q++) { c->prof_ms[q] = 0; c->prof_n[q] = 0; c->prof_bytes[q] = 0; }
   return BES_OK;
 }
 
-int bes_get_profile(bes_ctx *c, double *total_ms, long *launches, int nkinds) {
+int bes_get_profile(bes_ctx *c, double *total_ms, long *launches, double *bytes, int nkinds) {
   if (!c || nkinds < 0) { set_error("invalid argument"); return BES_ERR_ARG; }
   int s = prof_collect(c);
   if (s) return s;
   for (int q = 0; q < nkinds && q < bes_ctx::kProfKinds; q++) {
     if (total_ms) total_ms[q] = c->prof_ms[q];
     if (launches) launches[q] = c->prof_n[q];
+    if (bytes) bytes[q] = c->prof_bytes[q];
   }
   return BES_OK;
 }

@@ -1,7 +1,1243 @@
-// Implicit path of the BES solver core (filled in below).
+// Implicit path of the BES solver core on MI355X (gfx950).
+//
+// One backward-Euler pseudo-time step (SolverImplicit.cpp:293-345):
+//   k_timestep + k_gradlim          (explicit.hip)
+//   k_face_jacobian   first-order flux Jacobians dF/dU_L, dF/dU_R per face by
+//                     one-direction dual numbers with the reference's Eigen AD
+//                     rules (FluxJacobianWrapper.tpp:32-82), one thread per
+//                     (face, input column)
+//   k_assemble        row-owned assembly: second-order fluxes give the right-
+//                     hand side b = -dt/V sum +-F*A; the block row gets
+//                     I + sum nV x (dt/V*A*J) in the reference's accumulation
+//                     order (SolverImplicit.cpp:358-507, CoefficientMatrix.cpp:
+//                     166-204, the nV-fold repeat included).  No atomics.
+//   GMRES(m)          right-preconditioned, classical Gram-Schmidt with DGKS
+//                     re-orthogonalisation, deterministic two-pass reductions;
+//                     replaces the reference's AMGCL solve (CoefficientMatrix.cpp:
+//                     210-325) with relative tolerance semantics (||r|| <= tol*||b||)
+//   preconditioners   block Jacobi, or block ILU(0) in a multicolour order
+//                     (colours of the cell adjacency graph: every colour is
+//                     one parallel sweep of the triangular solves)
+//   k_update          U += x, the implicit update (SolverImplicit.cpp:509-528)
+// Matrix layout ("ELL-BSR"): slot 0 = diagonal block, slot 1+m = the block of
+// the neighbour through local face m; entries stored [slot][e][N] so a wave
+// reads 64 consecutive rows of one entry in one coalesced load.  Matrix and
+// Krylov vectors are FP64, or FP32 for the mixed-precision inner solve.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
 
 #include "../../include/bes_hip.h"
 #include "context.hpp"
+#include "flux.hpp"
+#include "kernels_common.hpp"
 
-extern "C" void bes_implicit_free(bes_ctx *c) { (void)c; }
+namespace bes {
+
+int launch_timestep(bes_ctx *c, const double *U, double cfl, int local);
+int launch_gradlim(bes_ctx *c, const double *U, double *Gout, double *Pout);
+int state_to_host(bes_ctx *c, const double *dsoa, int ncomp, double *host);
+int state_to_device(bes_ctx *c, const double *host, int ncomp, double *dsoa);
+
+constexpr int kRedBlocks = 1024;   // partial-sum blocks of the reductions
+
+struct FaceDev {
+  int F = 0;
+  int *fo = nullptr;       // [F] owner (device order)
+  int *fn = nullptr;       // [F] neighbour, -1 slip wall, -2 far field
+  double *fnrm = nullptr;  // [nd][F] unit normal
+  int *cf = nullptr;       // [k][N] face id of (cell, local face)
+  int *mrev = nullptr;     // [k][N] local index of this face in the neighbour
+};
+
+template <typename TM>
+struct Mat {
+  TM *val = nullptr;       // [(k+1)*nv*nv][N]
+};
+
+struct Impl {
+  int prec = 64;           // 64 or 32
+  int precond = 2;         // 0 none, 1 block Jacobi, 2 multicolour ILU(0)
+  int restart = 30, maxiter = 1000;
+  double rtol = 1e-7;
+  int repeat_nv = 1;
+  FaceDev fd;
+  double *JF = nullptr;    // [F][2][nv*nv]
+  double *b = nullptr;     // [N][nv] FP64 right-hand side
+  double *x = nullptr;     // [N][nv] FP64 solution
+  void *A = nullptr;       // matrix values (TM)
+  void *LU = nullptr;      // factor / inverse diagonal blocks (TM), [(k+1)*nv*nv][N]
+  void *V = nullptr;       // Krylov basis (TV) [(m+1)][N*nv]
+  void *Z = nullptr;       // work vectors (TV) [3][N*nv]
+  double *part = nullptr;  // reduction partials [32][kRedBlocks]
+  double *dots = nullptr;  // [64]
+  unsigned long long *bits = nullptr;   // [16] Linf monitors
+  int ncolor = 0;
+  std::vector<int> color_start;   // host: rows of colour c are crow[color_start[c] .. color_start[c+1])
+  int *crow = nullptr;            // [N] rows grouped by colour
+  int *color = nullptr;           // [N]
+  long ninternal = 0;              // internal faces (off-diagonal block pairs)
+  int last_iters = 0;
+  double last_relres = 0;
+  bool assembled = false, factored = false;
+};
+
+// ---------------------------------------------------------------------------
+// Jacobians
+// ---------------------------------------------------------------------------
+struct FinfI {
+  double p[6];
+};
+
+template <int ND>
+__global__ __launch_bounds__(256) void k_face_jacobian(const double *__restrict__ U, int N, FaceDev fd,
+                                                       GasConst gc, FinfI fi, double *__restrict__ JF) {
+  constexpr int NV = ND + 2, NB = NV * NV;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int f = t / (2 * NV), dir = t % (2 * NV);
+  if (f >= fd.F) return;
+  const int o = fd.fo[f], j = fd.fn[f];
+  if (j < 0 && dir >= NV) return;   // boundary faces: left Jacobian only
+  double nn[ND];
+#pragma unroll
+  for (int d = 0; d < ND; d++) nn[d] = fd.fnrm[(size_t)d * fd.F + f];
+  Dual ul[NV], ur[NV + 1], F[NV];
+#pragma unroll
+  for (int v = 0; v < NV; v++) ul[v] = Dual(U[(size_t)v * N + o], dir == v ? 1.0 : 0.0);
+  if (j >= 0) {
+#pragma unroll
+    for (int v = 0; v < NV; v++) ur[v] = Dual(U[(size_t)v * N + j], dir == NV + v ? 1.0 : 0.0);
+    roe_flux<ND, Dual>(gc, ul, ur, nn, F);
+  } else if (j == -1) {
+    slip_flux<ND, Dual>(gc, ul, nn, F);
+  } else {
+#pragma unroll
+    for (int v = 0; v < NV + 1; v++) ur[v] = Dual(fi.p[v]);
+    freestream_flux<ND, Dual>(gc, ul, ur, nn, F);
+  }
+  const int side = dir < NV ? 0 : 1, col = dir < NV ? dir : dir - NV;
+  double *J = JF + ((size_t)f * 2 + side) * NB;
+#pragma unroll
+  for (int r = 0; r < NV; r++) J[r * NV + col] = F[r].d;
+}
+
+// ---------------------------------------------------------------------------
+// assembly: right-hand side from second-order fluxes, blocks from JF
+// ---------------------------------------------------------------------------
+template <int ND, typename TM>
+__global__ __launch_bounds__(256) void k_assemble(
+    const double *__restrict__ U, const double *__restrict__ LG, const double *__restrict__ dt,
+    const double *__restrict__ dtg, DevMesh dm, GasConst gc, FinfI fi, FaceDev fd,
+    const double *__restrict__ JF, int repeat, double *__restrict__ b, TM *__restrict__ A,
+    unsigned long long *__restrict__ linf_b, unsigned long long *__restrict__ linf_r,
+    unsigned long long *__restrict__ err) {
+  constexpr int NV = ND + 2, K = ND + 1, NB = NV * NV;
+  const int N = dm.N;
+  const int i = cell_index();
+  double bi[NV], ri[NV];
+#pragma unroll
+  for (int v = 0; v < NV; v++) { bi[v] = 0.0; ri[v] = 0.0; }
+  if (i < N) {
+    const double mdt = (dtg ? dtg[0] : dt[i]) / dm.vol[i];
+    const uint32_t info = dm.info[i];
+    double u[NV], lg[NV][ND];
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+      u[v] = U[(size_t)v * N + i];
+#pragma unroll
+      for (int d = 0; d < ND; d++) lg[v][d] = LG[((size_t)v * ND + d) * N + i];
+    }
+    // right-hand side (SolverImplicit.cpp:441-473), faces in reference order
+    for (int q = 0; q < K; q++) {
+      const int m = (info >> (2 * q)) & 3;
+      const bool own = (info >> (8 + m)) & 1;
+      const int j = dm.nbr[(size_t)m * N + i];
+      double nn[ND], rs[ND];
+#pragma unroll
+      for (int d = 0; d < ND; d++) {
+        nn[d] = dm.nf[((size_t)m * ND + d) * N + i];
+        rs[d] = dm.rs[((size_t)m * ND + d) * N + i];
+      }
+      const double Af = dm.af[(size_t)m * N + i];
+      double ui[NV], F[NV];
+#pragma unroll
+      for (int v = 0; v < NV; v++) {
+        ui[v] = u[v];
+#pragma unroll
+        for (int d = 0; d < ND; d++) ui[v] += lg[v][d] * rs[d];
+      }
+      if (j >= 0) {
+        double rn[ND], uj[NV];
+#pragma unroll
+        for (int d = 0; d < ND; d++) rn[d] = dm.rn[((size_t)m * ND + d) * N + i];
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+          uj[v] = U[(size_t)v * N + j];
+#pragma unroll
+          for (int d = 0; d < ND; d++) uj[v] += LG[((size_t)v * ND + d) * N + j] * rn[d];
+        }
+        if (own) roe_flux<ND, double>(gc, ui, uj, nn, F);
+        else roe_flux<ND, double>(gc, uj, ui, nn, F);
+      } else if (j == -1) {
+        slip_flux<ND, double>(gc, ui, nn, F);
+      } else {
+        freestream_flux<ND, double>(gc, ui, fi.p, nn, F);
+      }
+      bool bad = false;
+#pragma unroll
+      for (int v = 0; v < NV; v++) bad |= (F[v] != F[v]);
+      if (bad) atomicMin(err, (unsigned long long)i * K + m);
+      const double mul = mdt * Af;
+#pragma unroll
+      for (int v = 0; v < NV; v++) {
+        if (own) { bi[v] -= mul * F[v]; ri[v] += F[v] * Af; }
+        else { bi[v] += mdt * Af * F[v]; ri[v] -= F[v] * Af; }
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < NV; v++) b[(size_t)i * NV + v] = bi[v];
+    // blocks: identity, then per face (reference order) the repeated adds
+    double D[NB];
+#pragma unroll
+    for (int e = 0; e < NB; e++) D[e] = 0.0;
+#pragma unroll
+    for (int a = 0; a < NV; a++) D[a * NV + a] += 1.0;
+    for (int q = 0; q < K; q++) {
+      const int m = (info >> (2 * q)) & 3;
+      const bool own = (info >> (8 + m)) & 1;
+      const int j = dm.nbr[(size_t)m * N + i];
+      const int f = fd.cf[(size_t)m * N + i];
+      const double Af = dm.af[(size_t)m * N + i];
+      const double mul = own ? mdt * Af : -(mdt * Af);
+      // own row: diag += mul*JL, off(i,nbr) = mul*JR; neighbour row:
+      // diag += (-m)*JR, off(i,owner) = (-m)*JL
+      const double *Jd = JF + ((size_t)f * 2 + (own ? 0 : 1)) * NB;
+      const double *Jo = JF + ((size_t)f * 2 + (own ? 1 : 0)) * NB;
+#pragma unroll
+      for (int e = 0; e < NB; e++) {
+        const double x = mul * Jd[e];
+        for (int r = 0; r < repeat; r++) D[e] += x;
+      }
+      TM *Ao = A + (size_t)(1 + m) * NB * N;
+      if (j >= 0) {
+#pragma unroll
+        for (int e = 0; e < NB; e++) {
+          const double x = mul * Jo[e];
+          double s = 0.0;
+          for (int r = 0; r < repeat; r++) s += x;
+          Ao[(size_t)e * N + i] = (TM)s;
+        }
+      } else {
+#pragma unroll
+        for (int e = 0; e < NB; e++) Ao[(size_t)e * N + i] = (TM)0;
+      }
+    }
+#pragma unroll
+    for (int e = 0; e < NB; e++) A[(size_t)e * N + i] = (TM)D[e];
+  }
+  if (linf_b) {
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+      double a = block_max(::fabs(bi[v]));
+      if (threadIdx.x == 0) atomicMax(&linf_b[v], (unsigned long long)__double_as_longlong(a));
+      double c = block_max(::fabs(ri[v]));
+      if (threadIdx.x == 0) atomicMax(&linf_r[v], (unsigned long long)__double_as_longlong(c));
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// small dense block helpers (NV x NV, row-major, registers)
+// ---------------------------------------------------------------------------
+template <int NV>
+__device__ __forceinline__ void block_inverse(double *A, double *Inv) {
+  // Gauss-Jordan with partial pivoting
+#pragma unroll
+  for (int r = 0; r < NV; r++)
+#pragma unroll
+    for (int c = 0; c < NV; c++) Inv[r * NV + c] = (r == c) ? 1.0 : 0.0;
+#pragma unroll
+  for (int p = 0; p < NV; p++) {
+    int piv = p;
+    double best = ::fabs(A[p * NV + p]);
+#pragma unroll
+    for (int r = p + 1; r < NV; r++) {
+      double a = ::fabs(A[r * NV + p]);
+      if (a > best) { best = a; piv = r; }
+    }
+    if (piv != p) {
+#pragma unroll
+      for (int c = 0; c < NV; c++) {
+        double t = A[p * NV + c]; A[p * NV + c] = A[piv * NV + c]; A[piv * NV + c] = t;
+        t = Inv[p * NV + c]; Inv[p * NV + c] = Inv[piv * NV + c]; Inv[piv * NV + c] = t;
+      }
+    }
+    const double inv = 1.0 / A[p * NV + p];
+#pragma unroll
+    for (int c = 0; c < NV; c++) { A[p * NV + c] *= inv; Inv[p * NV + c] *= inv; }
+#pragma unroll
+    for (int r = 0; r < NV; r++) {
+      if (r == p) continue;
+      const double f = A[r * NV + p];
+#pragma unroll
+      for (int c = 0; c < NV; c++) {
+        A[r * NV + c] -= f * A[p * NV + c];
+        Inv[r * NV + c] -= f * Inv[p * NV + c];
+      }
+    }
+  }
+}
+
+template <typename TM, int NB>
+__device__ __forceinline__ void load_block(const TM *__restrict__ base, int slot, int N, int i, double *B) {
+#pragma unroll
+  for (int e = 0; e < NB; e++) B[e] = (double)base[((size_t)slot * NB + e) * N + i];
+}
+template <typename TM, int NB>
+__device__ __forceinline__ void store_block(TM *__restrict__ base, int slot, int N, int i, const double *B) {
+#pragma unroll
+  for (int e = 0; e < NB; e++) base[((size_t)slot * NB + e) * N + i] = (TM)B[e];
+}
+
+// block Jacobi: LU slot 0 <- inverse of the diagonal block
+template <int ND, typename TM>
+__global__ __launch_bounds__(256) void k_bj_setup(const TM *__restrict__ A, TM *__restrict__ LU, int N) {
+  constexpr int NV = ND + 2, NB = NV * NV;
+  const int i = cell_index();
+  if (i >= N) return;
+  double D[NB], Inv[NB];
+  load_block<TM, NB>(A, 0, N, i, D);
+  block_inverse<NV>(D, Inv);
+  store_block<TM, NB>(LU, 0, N, i, Inv);
+}
+
+// block ILU(0) factorisation of the rows of one colour (all lower colours done).
+// LU holds a copy of A on entry; on exit slot 0 holds inv(U_ii), lower slots L_ik,
+// upper slots U_ij.
+template <int ND, typename TM>
+__global__ __launch_bounds__(256) void k_ilu_factor_color(TM *__restrict__ LU, DevMesh dm, FaceDev fd,
+                                                          const int *__restrict__ color,
+                                                          const int *__restrict__ rows, int nrows) {
+  constexpr int NV = ND + 2, K = ND + 1, NB = NV * NV;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nrows) return;
+  const int N = dm.N;
+  const int i = rows[t];
+  const int ci = color[i];
+  int nb[K], cn[K];
+#pragma unroll
+  for (int m = 0; m < K; m++) {
+    nb[m] = dm.nbr[(size_t)m * N + i];
+    cn[m] = nb[m] >= 0 ? color[nb[m]] : 0x7fffffff;
+  }
+  // lower neighbours in elimination order (colour, then row index)
+  int ord[K];
+  int nl = 0;
+#pragma unroll
+  for (int m = 0; m < K; m++)
+    if (nb[m] >= 0 && cn[m] < ci) ord[nl++] = m;
+  for (int a = 1; a < nl; a++)
+    for (int bq = a; bq > 0; bq--) {
+      const int x = ord[bq - 1], y = ord[bq];
+      const bool swap = cn[x] > cn[y] || (cn[x] == cn[y] && nb[x] > nb[y]);
+      if (swap) { ord[bq - 1] = y; ord[bq] = x; }
+    }
+  double Aii[NB];
+  load_block<TM, NB>(LU, 0, N, i, Aii);
+  for (int q = 0; q < nl; q++) {
+    const int m = ord[q];
+    const int k = nb[m];
+    double Lik[NB], Uinv[NB], Aik[NB];
+    load_block<TM, NB>(LU, 1 + m, N, i, Aik);
+    load_block<TM, NB>(LU, 0, N, k, Uinv);   // row k is final: slot 0 = inv(U_kk)
+#pragma unroll
+    for (int r = 0; r < NV; r++)
+#pragma unroll
+      for (int c = 0; c < NV; c++) {
+        double s = 0;
+#pragma unroll
+        for (int p = 0; p < NV; p++) s += Aik[r * NV + p] * Uinv[p * NV + c];
+        Lik[r * NV + c] = s;
+      }
+    store_block<TM, NB>(LU, 1 + m, N, i, Lik);
+    // A_ij -= L_ik U_kj for j in row k's upper part that are in row i's pattern
+#pragma unroll
+    for (int mk = 0; mk < K; mk++) {
+      const int jk = dm.nbr[(size_t)mk * N + k];
+      if (jk < 0) continue;
+      const int cj = color[jk];
+      if (!(cj > cn[m] || (cj == cn[m] && jk > k))) continue;   // not upper in row k
+      int target = -1;   // -2 = diagonal of row i
+      if (jk == i) target = -2;
+      else {
+#pragma unroll
+        for (int mm = 0; mm < K; mm++) if (nb[mm] == jk) target = mm;
+      }
+      if (target == -1) continue;
+      double Ukj[NB];
+      load_block<TM, NB>(LU, 1 + mk, N, k, Ukj);
+      double upd[NB];
+#pragma unroll
+      for (int r = 0; r < NV; r++)
+#pragma unroll
+        for (int c = 0; c < NV; c++) {
+          double s = 0;
+#pragma unroll
+          for (int p = 0; p < NV; p++) s += Lik[r * NV + p] * Ukj[p * NV + c];
+          upd[r * NV + c] = s;
+        }
+      if (target == -2) {
+#pragma unroll
+        for (int e = 0; e < NB; e++) Aii[e] -= upd[e];
+      } else {
+        double Aij[NB];
+        load_block<TM, NB>(LU, 1 + target, N, i, Aij);
+#pragma unroll
+        for (int e = 0; e < NB; e++) Aij[e] -= upd[e];
+        store_block<TM, NB>(LU, 1 + target, N, i, Aij);
+      }
+    }
+  }
+  double Inv[NB];
+  block_inverse<NV>(Aii, Inv);
+  store_block<TM, NB>(LU, 0, N, i, Inv);
+}
+
+// ---------------------------------------------------------------------------
+// Krylov kernels (vectors [N][nv], type TV)
+// ---------------------------------------------------------------------------
+template <int ND, typename TM, typename TV>
+__global__ __launch_bounds__(256) void k_spmv(const TM *__restrict__ A, DevMesh dm, const TV *__restrict__ x,
+                                              TV *__restrict__ y) {
+  constexpr int NV = ND + 2, K = ND + 1, NB = NV * NV;
+  const int N = dm.N;
+  const int i = cell_index();
+  if (i >= N) return;
+  double acc[NV];
+  double xi[NV];
+#pragma unroll
+  for (int c = 0; c < NV; c++) xi[c] = (double)x[(size_t)i * NV + c];
+#pragma unroll
+  for (int r = 0; r < NV; r++) {
+    double s = 0;
+#pragma unroll
+    for (int c = 0; c < NV; c++) s += (double)A[((size_t)(r * NV + c)) * N + i] * xi[c];
+    acc[r] = s;
+  }
+#pragma unroll
+  for (int m = 0; m < K; m++) {
+    const int j = dm.nbr[(size_t)m * N + i];
+    if (j < 0) continue;
+    double xj[NV];
+#pragma unroll
+    for (int c = 0; c < NV; c++) xj[c] = (double)x[(size_t)j * NV + c];
+    const TM *Ao = A + (size_t)(1 + m) * NB * N;
+#pragma unroll
+    for (int r = 0; r < NV; r++) {
+      double s = 0;
+#pragma unroll
+      for (int c = 0; c < NV; c++) s += (double)Ao[((size_t)(r * NV + c)) * N + i] * xj[c];
+      acc[r] += s;
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < NV; r++) y[(size_t)i * NV + r] = (TV)acc[r];
+}
+
+// z = inv(D_i) r_i (block Jacobi; also the diagonal step of the U solve)
+template <int ND, typename TM, typename TV>
+__global__ __launch_bounds__(256) void k_bj_apply(const TM *__restrict__ LU, int N, const TV *__restrict__ r,
+                                                  TV *__restrict__ z) {
+  constexpr int NV = ND + 2;
+  const int i = cell_index();
+  if (i >= N) return;
+  double ri[NV];
+#pragma unroll
+  for (int c = 0; c < NV; c++) ri[c] = (double)r[(size_t)i * NV + c];
+#pragma unroll
+  for (int a = 0; a < NV; a++) {
+    double s = 0;
+#pragma unroll
+    for (int c = 0; c < NV; c++) s += (double)LU[((size_t)(a * NV + c)) * N + i] * ri[c];
+    z[(size_t)i * NV + a] = (TV)s;
+  }
+}
+
+// forward sweep of one colour: y_i = r_i - sum_{k lower} L_ik y_k
+template <int ND, typename TM, typename TV>
+__global__ __launch_bounds__(256) void k_ilu_lower(const TM *__restrict__ LU, DevMesh dm,
+                                                   const int *__restrict__ color, const int *__restrict__ rows,
+                                                   int nrows, const TV *__restrict__ r, TV *__restrict__ y) {
+  constexpr int NV = ND + 2, K = ND + 1, NB = NV * NV;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nrows) return;
+  const int N = dm.N;
+  const int i = rows[t];
+  const int ci = color[i];
+  double acc[NV];
+#pragma unroll
+  for (int c = 0; c < NV; c++) acc[c] = (double)r[(size_t)i * NV + c];
+#pragma unroll
+  for (int m = 0; m < K; m++) {
+    const int k = dm.nbr[(size_t)m * N + i];
+    if (k < 0 || color[k] >= ci) continue;
+    double yk[NV];
+#pragma unroll
+    for (int c = 0; c < NV; c++) yk[c] = (double)y[(size_t)k * NV + c];
+    const TM *L = LU + (size_t)(1 + m) * NB * N;
+#pragma unroll
+    for (int a = 0; a < NV; a++) {
+      double s = 0;
+#pragma unroll
+      for (int c = 0; c < NV; c++) s += (double)L[((size_t)(a * NV + c)) * N + i] * yk[c];
+      acc[a] -= s;
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < NV; c++) y[(size_t)i * NV + c] = (TV)acc[c];
+}
+
+// backward sweep of one colour: x_i = inv(U_ii) (y_i - sum_{j upper} U_ij x_j)
+template <int ND, typename TM, typename TV>
+__global__ __launch_bounds__(256) void k_ilu_upper(const TM *__restrict__ LU, DevMesh dm,
+                                                   const int *__restrict__ color, const int *__restrict__ rows,
+                                                   int nrows, const TV *__restrict__ y, TV *__restrict__ x) {
+  constexpr int NV = ND + 2, K = ND + 1, NB = NV * NV;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nrows) return;
+  const int N = dm.N;
+  const int i = rows[t];
+  const int ci = color[i];
+  double acc[NV];
+#pragma unroll
+  for (int c = 0; c < NV; c++) acc[c] = (double)y[(size_t)i * NV + c];
+#pragma unroll
+  for (int m = 0; m < K; m++) {
+    const int j = dm.nbr[(size_t)m * N + i];
+    if (j < 0 || color[j] <= ci) continue;
+    double xj[NV];
+#pragma unroll
+    for (int c = 0; c < NV; c++) xj[c] = (double)x[(size_t)j * NV + c];
+    const TM *Uo = LU + (size_t)(1 + m) * NB * N;
+#pragma unroll
+    for (int a = 0; a < NV; a++) {
+      double s = 0;
+#pragma unroll
+      for (int c = 0; c < NV; c++) s += (double)Uo[((size_t)(a * NV + c)) * N + i] * xj[c];
+      acc[a] -= s;
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < NV; a++) {
+    double s = 0;
+#pragma unroll
+    for (int c = 0; c < NV; c++) s += (double)LU[((size_t)(a * NV + c)) * N + i] * acc[c];
+    x[(size_t)i * NV + a] = (TV)s;
+  }
+}
+
+// partial dot products of w with up to 8 basis vectors V[i0 .. i0+cnt), plus ||w||^2
+template <typename TV>
+__global__ __launch_bounds__(256) void k_mdot(const TV *__restrict__ V, size_t stride, int i0, int cnt,
+                                              const TV *__restrict__ w, size_t n, double *__restrict__ part) {
+  double acc[9];
+#pragma unroll
+  for (int q = 0; q < 9; q++) acc[q] = 0;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+    const double we = (double)w[e];
+#pragma unroll
+    for (int q = 0; q < 8; q++)
+      if (q < cnt) acc[q] += (double)V[(size_t)(i0 + q) * stride + e] * we;
+    acc[8] += we * we;
+  }
+#pragma unroll
+  for (int q = 0; q < 9; q++) {
+    double s = wave_sum(acc[q]);
+    __shared__ double sh[9][4];
+    if ((threadIdx.x & 63) == 0) sh[q][threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) part[(size_t)q * gridDim.x + blockIdx.x] = sh[q][0] + sh[q][1] + sh[q][2] + sh[q][3];
+    __syncthreads();
+  }
+}
+
+// out[q] = sum over blocks of part[q][.], fixed order (deterministic)
+__global__ void k_reduce_parts(const double *__restrict__ part, int nblk, int nq, double *__restrict__ out) {
+  const int q = blockIdx.x;
+  if (q >= nq) return;
+  double s = 0;
+  for (int b = threadIdx.x; b < nblk; b += blockDim.x) s += part[(size_t)q * nblk + b];
+  __shared__ double sh[4];
+  s = wave_sum(s);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) out[q] = sh[0] + sh[1] + sh[2] + sh[3];
+}
+
+// w -= sum_q h[q] V[q] for q < cnt, returns partial ||w||^2
+template <typename TV>
+__global__ __launch_bounds__(256) void k_orth_update(const TV *__restrict__ V, size_t stride, int cnt,
+                                                     const double *__restrict__ h, TV *__restrict__ w, size_t n,
+                                                     double *__restrict__ part) {
+  double nrm = 0;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+    double we = (double)w[e];
+    for (int q = 0; q < cnt; q++) we -= h[q] * (double)V[(size_t)q * stride + e];
+    w[e] = (TV)we;
+    const double wr = (double)w[e];
+    nrm += wr * wr;
+  }
+  double s = wave_sum(nrm);
+  __shared__ double sh[4];
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
+}
+
+// y = a*x (scale/copy with conversion)
+template <typename TA, typename TB>
+__global__ void k_scale(const TA *__restrict__ x, double a, TB *__restrict__ y, size_t n) {
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
+    y[e] = (TB)(a * (double)x[e]);
+}
+
+// u = sum_q c[q] V[q]
+template <typename TV>
+__global__ void k_combine(const TV *__restrict__ V, size_t stride, int cnt, const double *__restrict__ c,
+                          TV *__restrict__ u, size_t n) {
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+    double s = 0;
+    for (int q = 0; q < cnt; q++) s += c[q] * (double)V[(size_t)q * stride + e];
+    u[e] = (TV)s;
+  }
+}
+
+// x (FP64) += z ; r = b - y (residual with the FP64 right-hand side), partial ||r||^2
+template <typename TV>
+__global__ void k_axpy_x(double *__restrict__ x, const TV *__restrict__ z, size_t n) {
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
+    x[e] += (double)z[e];
+}
+template <typename TV>
+__global__ __launch_bounds__(256) void k_resid(const double *__restrict__ b, const TV *__restrict__ y,
+                                               TV *__restrict__ r, size_t n, double *__restrict__ part) {
+  double nrm = 0;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+    const double v = b[e] - (double)y[e];
+    r[e] = (TV)v;
+    nrm += v * v;
+  }
+  double s = wave_sum(nrm);
+  __shared__ double sh[4];
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
+}
+
+// implicit update U[v][i] += x[i][v]
+template <int ND>
+__global__ __launch_bounds__(256) void k_update(double *__restrict__ U, const double *__restrict__ x, int N) {
+  constexpr int NV = ND + 2;
+  const int i = cell_index();
+  if (i >= N) return;
+#pragma unroll
+  for (int v = 0; v < NV; v++) U[(size_t)v * N + i] += x[(size_t)i * NV + v];
+}
+
+__global__ void k_fill_u64_i(unsigned long long *p, int n, unsigned long long v) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+static Impl *impl_of(bes_ctx *c) { return (Impl *)c->impl; }
+
+static int impl_alloc(bes_ctx *c, Impl *im) {
+  const Geometry &g = c->geo;
+  const int N = c->N, k = c->k, nd = c->nd, nv = c->nv, NB = nv * nv;
+  const int F = (int)g.nface;
+  // faces in reference order with device cell ids
+  std::vector<int> fo(F), fn(F), cf((size_t)k * N), mrev((size_t)k * N, -1);
+  std::vector<double> fnrm((size_t)nd * F);
+  for (int f = 0; f < F; f++) {
+    fo[f] = (int)c->iperm[g.owner[f]];
+    if (g.nbr[f] >= 0) fn[f] = (int)c->iperm[g.nbr[f]];
+    else fn[f] = (g.btype[f] == kSlipWall) ? -1 : -2;
+    for (int d = 0; d < nd; d++) fnrm[(size_t)d * F + f] = g.n[f * nd + d];
+  }
+  for (int i = 0; i < N; i++) {
+    const int64_t e = c->perm[i];
+    for (int m = 0; m < k; m++) {
+      const int64_t f = g.cface[e * k + m];
+      cf[(size_t)m * N + i] = (int)f;
+      const int64_t j = g.cadj[e * k + m];
+      if (j >= 0)
+        for (int mm = 0; mm < k; mm++)
+          if (g.cface[j * k + mm] == f) mrev[(size_t)m * N + i] = mm;
+    }
+  }
+  // greedy colouring of the adjacency graph in device (Morton) order
+  std::vector<int> color(N, -1);
+  int ncol = 0;
+  for (int i = 0; i < N; i++) {
+    const int64_t e = c->perm[i];
+    unsigned used = 0;
+    for (int m = 0; m < k; m++) {
+      const int64_t j = g.cadj[e * k + m];
+      if (j >= 0 && color[c->iperm[j]] >= 0) used |= 1u << color[c->iperm[j]];
+    }
+    int col = 0;
+    while (used & (1u << col)) col++;
+    color[i] = col;
+    ncol = std::max(ncol, col + 1);
+  }
+  std::vector<int> crow;
+  crow.reserve(N);
+  im->color_start.assign(ncol + 1, 0);
+  for (int col = 0; col < ncol; col++) {
+    im->color_start[col] = (int)crow.size();
+    for (int i = 0; i < N; i++) if (color[i] == col) crow.push_back(i);
+  }
+  im->color_start[ncol] = (int)crow.size();
+  im->ncolor = ncol;
+  im->fd.F = F;
+  im->ninternal = 0;
+  for (int f = 0; f < F; f++) if (g.nbr[f] >= 0) im->ninternal++;
+  auto up = [&](auto **p, const auto &h) -> int {
+    using T = typename std::decay<decltype(h)>::type::value_type;
+    BES_HIP(hipMalloc((void **)p, std::max<size_t>(h.size(), 1) * sizeof(T)));
+    BES_HIP(hipMemcpy(*p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+    return BES_OK;
+  };
+  int s;
+  if ((s = up(&im->fd.fo, fo)) || (s = up(&im->fd.fn, fn)) || (s = up(&im->fd.fnrm, fnrm)) ||
+      (s = up(&im->fd.cf, cf)) || (s = up(&im->fd.mrev, mrev)) || (s = up(&im->color, color)) ||
+      (s = up(&im->crow, crow)))
+    return s;
+  const size_t nvec = (size_t)N * nv;
+  const size_t es = im->prec == 32 ? sizeof(float) : sizeof(double);
+  const size_t mat = (size_t)(k + 1) * NB * N;
+  BES_HIP(hipMalloc((void **)&im->JF, (size_t)F * 2 * NB * sizeof(double)));
+  BES_HIP(hipMemset(im->JF, 0, (size_t)F * 2 * NB * sizeof(double)));
+  BES_HIP(hipMalloc((void **)&im->b, nvec * sizeof(double)));
+  BES_HIP(hipMalloc((void **)&im->x, nvec * sizeof(double)));
+  BES_HIP(hipMalloc(&im->A, mat * es));
+  BES_HIP(hipMalloc(&im->LU, mat * es));
+  BES_HIP(hipMalloc(&im->V, (size_t)(im->restart + 1) * nvec * es));
+  BES_HIP(hipMalloc(&im->Z, 3 * nvec * es));
+  BES_HIP(hipMalloc((void **)&im->part, 32 * kRedBlocks * sizeof(double)));
+  BES_HIP(hipMalloc((void **)&im->dots, 64 * sizeof(double)));
+  BES_HIP(hipMalloc((void **)&im->bits, 16 * sizeof(unsigned long long)));
+  return BES_OK;
+}
+
+static void impl_release(Impl *im) {
+  void *ptrs[] = {im->fd.fo, im->fd.fn, im->fd.fnrm, im->fd.cf, im->fd.mrev, im->color, im->crow, im->JF,
+                  im->b, im->x, im->A, im->LU, im->V, im->Z, im->part, im->dots, im->bits};
+  for (void *p : ptrs)
+    if (p) hipFree(p);
+  delete im;
+}
+
+template <typename TM>
+static int launch_assemble(bes_ctx *c, Impl *im) {
+  const int bs = 256, nb = grid_for(c->N, bs);
+  FinfI fi;
+  std::memcpy(fi.p, c->finf, sizeof(fi.p));
+  const double *dtg = c->dt_local ? nullptr : c->dtg;
+  const double nbl = (double)c->nv * c->nv;
+  {
+    // per face: both states, normal, ids; writes one or two nv x nv blocks
+    ProfScope ps(c, kProfJacobian, (double)im->fd.F * (16.0 * c->nv + 8.0 * c->nd + 8.0) +
+                                       (double)(im->fd.F + im->ninternal) * nbl * 8.0);
+    const int nt = im->fd.F * 2 * c->nv;
+    if (c->nd == 2)
+      hipLaunchKernelGGL(k_face_jacobian<2>, dim3(grid_for(nt, bs)), dim3(bs), 0, c->stream, c->U0, c->N, im->fd,
+                         c->gc, fi, im->JF);
+    else
+      hipLaunchKernelGGL(k_face_jacobian<3>, dim3(grid_for(nt, bs)), dim3(bs), 0, c->stream, c->U0, c->N, im->fd,
+                         c->gc, fi, im->JF);
+    BES_HIP(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_fill_u64_i, dim3(1), dim3(64), 0, c->stream, im->bits, 16, 0ull);
+  // per cell: state, limited gradient, dt, volume, face records, both Jacobian
+  // blocks of each face; writes b and k+1 matrix blocks
+  ProfScope ps(c, kProfAssemble,
+               (double)c->N * (8.0 * c->nv + 8.0 * c->nv * c->nd + 20.0 + c->k * (16.0 + 24.0 * c->nd) +
+                               8.0 * c->nv) +
+                   (double)(im->fd.F + im->ninternal) * 2.0 * nbl * 8.0 +
+                   (double)c->N * (c->k + 1) * nbl * sizeof(TM));
+  if (c->nd == 2)
+    hipLaunchKernelGGL((k_assemble<2, TM>), dim3(nb), dim3(bs), 0, c->stream, c->U0, c->LG, c->dt, dtg, c->dm,
+                       c->gc, fi, im->fd, im->JF, im->repeat_nv, im->b, (TM *)im->A, im->bits, im->bits + 8,
+                       c->err);
+  else
+    hipLaunchKernelGGL((k_assemble<3, TM>), dim3(nb), dim3(bs), 0, c->stream, c->U0, c->LG, c->dt, dtg, c->dm,
+                       c->gc, fi, im->fd, im->JF, im->repeat_nv, im->b, (TM *)im->A, im->bits, im->bits + 8,
+                       c->err);
+  BES_HIP(hipGetLastError());
+  return BES_OK;
+}
+
+template <typename TM>
+static int factor(bes_ctx *c, Impl *im) {
+  const int bs = 256, N = c->N, nv = c->nv;
+  const size_t mat = (size_t)(c->k + 1) * nv * nv * N;
+  const double used = (double)(N + 2 * im->ninternal) * nv * nv * sizeof(TM);   // stored blocks
+  ProfScope ps(c, kProfFactor, im->precond == 1 ? 2.0 * N * nv * nv * sizeof(TM) : 3.0 * used);
+  if (im->precond == 1) {
+    if (c->nd == 2) hipLaunchKernelGGL((k_bj_setup<2, TM>), dim3(grid_for(N, bs)), dim3(bs), 0, c->stream, (const TM *)im->A, (TM *)im->LU, N);
+    else hipLaunchKernelGGL((k_bj_setup<3, TM>), dim3(grid_for(N, bs)), dim3(bs), 0, c->stream, (const TM *)im->A, (TM *)im->LU, N);
+  } else if (im->precond == 2) {
+    BES_HIP(hipMemcpyAsync(im->LU, im->A, mat * sizeof(TM), hipMemcpyDeviceToDevice, c->stream));
+    for (int col = 0; col < im->ncolor; col++) {
+      const int r0 = im->color_start[col], nr = im->color_start[col + 1] - r0;
+      if (nr == 0) continue;
+      if (c->nd == 2)
+        hipLaunchKernelGGL((k_ilu_factor_color<2, TM>), dim3(grid_for(nr, bs)), dim3(bs), 0, c->stream, (TM *)im->LU,
+                           c->dm, im->fd, im->color, im->crow + r0, nr);
+      else
+        hipLaunchKernelGGL((k_ilu_factor_color<3, TM>), dim3(grid_for(nr, bs)), dim3(bs), 0, c->stream, (TM *)im->LU,
+                           c->dm, im->fd, im->color, im->crow + r0, nr);
+    }
+  }
+  BES_HIP(hipGetLastError());
+  return BES_OK;
+}
+
+// z = M^-1 r
+template <typename TM, typename TV>
+static int precond_apply(bes_ctx *c, Impl *im, const TV *r, TV *z, TV *tmp) {
+  const int bs = 256, N = c->N;
+  const size_t n = (size_t)N * c->nv;
+  const double vb = (double)n * sizeof(TV), blocks = (double)(N + 2 * im->ninternal) * c->nv * c->nv;
+  // ILU: every stored block once over both sweeps, r, y (write+read), z,
+  // neighbour ids and colours; Jacobi: diagonal inverse, r, z
+  ProfScope ps(c, kProfPrecond,
+               im->precond == 2 ? blocks * sizeof(TM) + 4.0 * vb + 2.0 * N * (4.0 * c->k + 4.0)
+                                : (im->precond == 1 ? (double)N * c->nv * c->nv * sizeof(TM) + 2.0 * vb : 2.0 * vb));
+  if (im->precond == 0) {
+    hipLaunchKernelGGL((k_scale<TV, TV>), dim3(kRedBlocks), dim3(256), 0, c->stream, r, 1.0, z, n);
+  } else if (im->precond == 1) {
+    if (c->nd == 2) hipLaunchKernelGGL((k_bj_apply<2, TM, TV>), dim3(grid_for(N, bs)), dim3(bs), 0, c->stream, (const TM *)im->LU, N, r, z);
+    else hipLaunchKernelGGL((k_bj_apply<3, TM, TV>), dim3(grid_for(N, bs)), dim3(bs), 0, c->stream, (const TM *)im->LU, N, r, z);
+  } else {
+    for (int col = 0; col < im->ncolor; col++) {
+      const int r0 = im->color_start[col], nr = im->color_start[col + 1] - r0;
+      if (nr == 0) continue;
+      if (c->nd == 2)
+        hipLaunchKernelGGL((k_ilu_lower<2, TM, TV>), dim3(grid_for(nr, bs)), dim3(bs), 0, c->stream, (const TM *)im->LU,
+                           c->dm, im->color, im->crow + r0, nr, r, tmp);
+      else
+        hipLaunchKernelGGL((k_ilu_lower<3, TM, TV>), dim3(grid_for(nr, bs)), dim3(bs), 0, c->stream, (const TM *)im->LU,
+                           c->dm, im->color, im->crow + r0, nr, r, tmp);
+    }
+    for (int col = im->ncolor - 1; col >= 0; col--) {
+      const int r0 = im->color_start[col], nr = im->color_start[col + 1] - r0;
+      if (nr == 0) continue;
+      if (c->nd == 2)
+        hipLaunchKernelGGL((k_ilu_upper<2, TM, TV>), dim3(grid_for(nr, bs)), dim3(bs), 0, c->stream, (const TM *)im->LU,
+                           c->dm, im->color, im->crow + r0, nr, tmp, z);
+      else
+        hipLaunchKernelGGL((k_ilu_upper<3, TM, TV>), dim3(grid_for(nr, bs)), dim3(bs), 0, c->stream, (const TM *)im->LU,
+                           c->dm, im->color, im->crow + r0, nr, tmp, z);
+    }
+  }
+  BES_HIP(hipGetLastError());
+  return BES_OK;
+}
+
+template <typename TM, typename TV>
+static int spmv(bes_ctx *c, Impl *im, const TV *x, TV *y) {
+  const int bs = 256;
+  // stored blocks, x and y once, neighbour ids
+  ProfScope ps(c, kProfSpmv, (double)(c->N + 2 * im->ninternal) * c->nv * c->nv * sizeof(TM) +
+                                 2.0 * c->N * c->nv * sizeof(TV) + 4.0 * c->N * c->k);
+  if (c->nd == 2) hipLaunchKernelGGL((k_spmv<2, TM, TV>), dim3(grid_for(c->N, bs)), dim3(bs), 0, c->stream, (const TM *)im->A, c->dm, x, y);
+  else hipLaunchKernelGGL((k_spmv<3, TM, TV>), dim3(grid_for(c->N, bs)), dim3(bs), 0, c->stream, (const TM *)im->A, c->dm, x, y);
+  BES_HIP(hipGetLastError());
+  return BES_OK;
+}
+
+// dots h[0..cnt) = V[0..cnt)^T w and ||w||^2 (last), copied to host
+template <typename TV>
+static int multidot(bes_ctx *c, Impl *im, const TV *V, size_t stride, int cnt, const TV *w, size_t n, double *h,
+                    double *wnorm2) {
+  // chunk q (8 dots + ||w||^2) reduces into dots[9q .. 9q+8]; at most 4 chunks
+  const int nchunk = std::max(1, (cnt + 7) / 8);
+  if (nchunk > 4) { set_error("too many basis vectors"); return BES_ERR_ARG; }
+  {
+    ProfScope ps(c, kProfKrylov, (double)(cnt + nchunk) * n * sizeof(TV));
+    for (int q = 0; q < nchunk; q++) {
+      const int i0 = q * 8, cc = std::min(8, cnt - i0);
+      hipLaunchKernelGGL((k_mdot<TV>), dim3(kRedBlocks), dim3(256), 0, c->stream, V, stride, i0, std::max(cc, 0), w,
+                         n, im->part);
+      hipLaunchKernelGGL(k_reduce_parts, dim3(9), dim3(256), 0, c->stream, im->part, kRedBlocks, 9, im->dots + 9 * q);
+    }
+  }
+  BES_HIP(hipGetLastError());
+  double tmp[36];
+  BES_HIP(hipMemcpyAsync(tmp, im->dots, sizeof(double) * 9 * nchunk, hipMemcpyDeviceToHost, c->stream));
+  BES_HIP(hipStreamSynchronize(c->stream));
+  for (int q = 0; q < cnt; q++) h[q] = tmp[9 * (q / 8) + (q % 8)];
+  if (wnorm2) *wnorm2 = tmp[8];
+  return BES_OK;
+}
+
+template <typename TM, typename TV>
+static int gmres(bes_ctx *c, Impl *im, int *iters_out, double *relres_out) {
+  const int m = im->restart;
+  const size_t n = (size_t)c->N * c->nv;
+  TV *V = (TV *)im->V;
+  TV *Z = (TV *)im->Z, *Z1 = Z + n, *Z2 = Z + 2 * n;
+  hipMemsetAsync(im->x, 0, n * sizeof(double), c->stream);
+  // ||b||
+  double bnorm2 = 0;
+  hipLaunchKernelGGL((k_scale<double, TV>), dim3(kRedBlocks), dim3(256), 0, c->stream, im->b, 1.0, Z, n);
+  {
+    double dummy;
+    int s = multidot<TV>(c, im, Z, n, 0, Z, n, &dummy, &bnorm2);
+    if (s) return s;
+  }
+  const double bnorm = std::sqrt(bnorm2);
+  *iters_out = 0;
+  *relres_out = 0;
+  if (bnorm == 0) return BES_OK;
+  const double target = im->rtol * bnorm;
+  std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), y(m), hcol(m + 1);
+  int total = 0;
+  double rnorm = bnorm;
+  bool first = true;
+  while (total < im->maxiter) {
+    // r = b - A x (x = 0 on the first cycle, as the reference starts every solve)
+    if (first) {
+      hipLaunchKernelGGL((k_scale<double, TV>), dim3(kRedBlocks), dim3(256), 0, c->stream, im->b, 1.0 / bnorm, V, n);
+      rnorm = bnorm;
+    } else {
+      // x in FP64 -> Z2 (TV), y = A Z2, r = b - y
+      hipLaunchKernelGGL((k_scale<double, TV>), dim3(kRedBlocks), dim3(256), 0, c->stream, im->x, 1.0, Z2, n);
+      int s = spmv<TM, TV>(c, im, Z2, Z1);
+      if (s) return s;
+      hipLaunchKernelGGL((k_resid<TV>), dim3(kRedBlocks), dim3(256), 0, c->stream, im->b, Z1, Z, n, im->part);
+      hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(256), 0, c->stream, im->part, kRedBlocks, 1, im->dots);
+      double r2;
+      BES_HIP(hipMemcpyAsync(&r2, im->dots, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+      BES_HIP(hipStreamSynchronize(c->stream));
+      rnorm = std::sqrt(r2);
+      if (rnorm <= target) break;
+      hipLaunchKernelGGL((k_scale<TV, TV>), dim3(kRedBlocks), dim3(256), 0, c->stream, Z, 1.0 / rnorm, V, n);
+    }
+    first = false;
+    std::fill(g.begin(), g.end(), 0.0);
+    g[0] = rnorm;
+    int j = 0;
+    bool done = false;
+    for (; j < m && total < im->maxiter; j++) {
+      TV *vj = V + (size_t)j * n, *w = V + (size_t)(j + 1) * n;
+      int s;
+      if ((s = precond_apply<TM, TV>(c, im, vj, Z, Z1))) return s;
+      if ((s = spmv<TM, TV>(c, im, Z, w))) return s;
+      // classical Gram-Schmidt with one DGKS re-orthogonalisation
+      double w0n2 = 0, wn2 = 0;
+      if ((s = multidot<TV>(c, im, V, n, j + 1, w, n, hcol.data(), &w0n2))) return s;
+      BES_HIP(hipMemcpyAsync(im->dots, hcol.data(), sizeof(double) * (j + 1), hipMemcpyHostToDevice, c->stream));
+      {
+        ProfScope ps(c, kProfKrylov, (double)(j + 3) * n * sizeof(TV));
+        hipLaunchKernelGGL((k_orth_update<TV>), dim3(kRedBlocks), dim3(256), 0, c->stream, V, n, j + 1, im->dots + 0, w,
+                           n, im->part);
+        hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(256), 0, c->stream, im->part, kRedBlocks, 1, im->dots + 40);
+      }
+      BES_HIP(hipMemcpyAsync(&wn2, im->dots + 40, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+      BES_HIP(hipStreamSynchronize(c->stream));
+      if (std::sqrt(wn2) < 0.7 * std::sqrt(w0n2)) {
+        std::vector<double> h2(j + 1);
+        if ((s = multidot<TV>(c, im, V, n, j + 1, w, n, h2.data(), nullptr))) return s;
+        for (int q = 0; q <= j; q++) hcol[q] += h2[q];
+        BES_HIP(hipMemcpyAsync(im->dots, h2.data(), sizeof(double) * (j + 1), hipMemcpyHostToDevice, c->stream));
+        {
+          ProfScope ps(c, kProfKrylov, (double)(j + 3) * n * sizeof(TV));
+          hipLaunchKernelGGL((k_orth_update<TV>), dim3(kRedBlocks), dim3(256), 0, c->stream, V, n, j + 1, im->dots + 0,
+                             w, n, im->part);
+          hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(256), 0, c->stream, im->part, kRedBlocks, 1, im->dots + 40);
+        }
+        BES_HIP(hipMemcpyAsync(&wn2, im->dots + 40, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        BES_HIP(hipStreamSynchronize(c->stream));
+      }
+      const double hn = std::sqrt(wn2);
+      hcol[j + 1] = hn;
+      if (hn > 0)
+        hipLaunchKernelGGL((k_scale<TV, TV>), dim3(kRedBlocks), dim3(256), 0, c->stream, w, 1.0 / hn, w, n);
+      // Givens rotations
+      for (int q = 0; q < j; q++) {
+        const double t = cs[q] * hcol[q] + sn[q] * hcol[q + 1];
+        hcol[q + 1] = -sn[q] * hcol[q] + cs[q] * hcol[q + 1];
+        hcol[q] = t;
+      }
+      const double den = std::hypot(hcol[j], hcol[j + 1]);
+      cs[j] = den > 0 ? hcol[j] / den : 1.0;
+      sn[j] = den > 0 ? hcol[j + 1] / den : 0.0;
+      hcol[j] = den;
+      hcol[j + 1] = 0;
+      g[j + 1] = -sn[j] * g[j];
+      g[j] = cs[j] * g[j];
+      for (int q = 0; q <= j; q++) H[(size_t)q * m + j] = hcol[q];
+      total++;
+      rnorm = std::fabs(g[j + 1]);
+      if (rnorm <= target || hn == 0) { j++; done = true; break; }
+    }
+    // solve H y = g (upper triangular j x j)
+    for (int q = j - 1; q >= 0; q--) {
+      double s = g[q];
+      for (int p = q + 1; p < j; p++) s -= H[(size_t)q * m + p] * y[p];
+      y[q] = s / H[(size_t)q * m + q];
+    }
+    BES_HIP(hipMemcpyAsync(im->dots, y.data(), sizeof(double) * j, hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL((k_combine<TV>), dim3(kRedBlocks), dim3(256), 0, c->stream, V, n, j, im->dots, Z2, n);
+    int s;
+    if ((s = precond_apply<TM, TV>(c, im, Z2, Z, Z1))) return s;
+    hipLaunchKernelGGL((k_axpy_x<TV>), dim3(kRedBlocks), dim3(256), 0, c->stream, im->x, Z, n);
+    BES_HIP(hipGetLastError());
+    if (done) break;
+  }
+  // true residual of the returned solution
+  {
+    hipLaunchKernelGGL((k_scale<double, TV>), dim3(kRedBlocks), dim3(256), 0, c->stream, im->x, 1.0, Z2, n);
+    int s = spmv<TM, TV>(c, im, Z2, Z1);
+    if (s) return s;
+    hipLaunchKernelGGL((k_resid<TV>), dim3(kRedBlocks), dim3(256), 0, c->stream, im->b, Z1, Z, n, im->part);
+    hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(256), 0, c->stream, im->part, kRedBlocks, 1, im->dots);
+    double r2;
+    BES_HIP(hipMemcpyAsync(&r2, im->dots, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    BES_HIP(hipStreamSynchronize(c->stream));
+    rnorm = std::sqrt(r2);
+  }
+  *iters_out = total;
+  *relres_out = rnorm / bnorm;
+  return BES_OK;
+}
+
+}  // namespace bes
+
+using namespace bes;
+
+extern "C" {
+
+void bes_implicit_free(bes_ctx *c) {
+  if (c && c->impl) {
+    impl_release((Impl *)c->impl);
+    c->impl = nullptr;
+  }
+}
+
+int bes_implicit_setup(bes_ctx *c, int precision, int precond, int restart, int maxiter, double rtol) {
+  if (!c || (precision != 64 && precision != 32) || precond < 0 || precond > 2 || restart < 1 || restart > 30 ||
+      maxiter < 1 || !(rtol > 0)) {
+    set_error("invalid implicit setup argument (precision 64/32, precond 0-2, restart 1-30)");
+    return BES_ERR_ARG;
+  }
+  bes_implicit_free(c);
+  Impl *im = new Impl();
+  im->prec = precision;
+  im->precond = precond;
+  im->restart = restart;
+  im->maxiter = maxiter;
+  im->rtol = rtol;
+  c->impl = im;
+  int s = impl_alloc(c, im);
+  if (s) { bes_implicit_free(c); return s; }
+  return BES_OK;
+}
+
+int bes_implicit_assemble(bes_ctx *c, double cfl, int local, int repeat_nv, double *linf_b, double *linf_r) {
+  if (!c || !c->impl) { set_error("call bes_implicit_setup first"); return BES_ERR_ARG; }
+  Impl *im = impl_of(c);
+  im->repeat_nv = repeat_nv ? c->nv : 1;
+  int s;
+  hipLaunchKernelGGL(k_fill_u64_i, dim3(1), dim3(64), 0, c->stream, c->err, 1, ~0ull);
+  if ((s = launch_timestep(c, c->U0, cfl, local))) return s;
+  c->dt_local = local != 0;
+  if ((s = launch_gradlim(c, c->U0, nullptr, nullptr))) return s;
+  if (im->prec == 64) s = launch_assemble<double>(c, im);
+  else s = launch_assemble<float>(c, im);
+  if (s) return s;
+  unsigned long long bb[16];
+  BES_HIP(hipMemcpyAsync(bb, im->bits, sizeof(bb), hipMemcpyDeviceToHost, c->stream));
+  BES_HIP(hipStreamSynchronize(c->stream));
+  for (int v = 0; v < c->nv; v++) {
+    if (linf_b) std::memcpy(&linf_b[v], &bb[v], 8);
+    if (linf_r) std::memcpy(&linf_r[v], &bb[8 + v], 8);
+  }
+  im->assembled = true;
+  im->factored = false;
+  unsigned long long e = 0;
+  BES_HIP(hipMemcpy(&e, c->err, sizeof(e), hipMemcpyDeviceToHost));
+  if (e != ~0ull) {
+    const int64_t cell = c->perm[e / c->k];
+    c->last_err_face = (long)c->geo.cface[cell * c->k + (int)(e % c->k)];
+    set_error("non-finite flux at face " + std::to_string(c->last_err_face));
+    return BES_ERR_NAN;
+  }
+  return BES_OK;
+}
+
+// reference CoefficientMatrix layout: scalar CSR rows (cell, var) in reference
+// cell order, each row = own block row then neighbour block rows in local face
+// order (CoefficientMatrix.cpp:127-160).  rowptr [ncell*nv+1], col/val [nnz].
+int bes_implicit_get_system(bes_ctx *c, long *rowptr, long *col, double *val, double *rhs) {
+  if (!c || !c->impl || !impl_of(c)->assembled) { set_error("system not assembled"); return BES_ERR_ARG; }
+  Impl *im = impl_of(c);
+  const int N = c->N, nv = c->nv, k = c->k, NB = nv * nv;
+  const size_t mat = (size_t)(k + 1) * NB * N;
+  std::vector<double> A(mat);
+  if (im->prec == 64) {
+    BES_HIP(hipMemcpy(A.data(), im->A, mat * sizeof(double), hipMemcpyDeviceToHost));
+  } else {
+    std::vector<float> Af(mat);
+    BES_HIP(hipMemcpy(Af.data(), im->A, mat * sizeof(float), hipMemcpyDeviceToHost));
+    for (size_t q = 0; q < mat; q++) A[q] = Af[q];
+  }
+  const Geometry &g = c->geo;
+  long nz = 0, row = 0;
+  for (int64_t e = 0; e < N; e++) {
+    const int i = (int)c->iperm[e];
+    for (int a = 0; a < nv; a++) {
+      if (rowptr) rowptr[row] = nz;
+      for (int bq = 0; bq < nv; bq++) {
+        if (col) col[nz] = e * nv + bq;
+        if (val) val[nz] = A[((size_t)0 * NB + a * nv + bq) * N + i];
+        nz++;
+      }
+      for (int m = 0; m < k; m++) {
+        const int64_t j = g.cadj[e * k + m];
+        if (j < 0) continue;
+        for (int bq = 0; bq < nv; bq++) {
+          if (col) col[nz] = j * nv + bq;
+          if (val) val[nz] = A[((size_t)(1 + m) * NB + a * nv + bq) * N + i];
+          nz++;
+        }
+      }
+      row++;
+    }
+  }
+  if (rowptr) rowptr[row] = nz;
+  if (rhs) {
+    std::vector<double> b((size_t)N * nv);
+    BES_HIP(hipMemcpy(b.data(), im->b, b.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int64_t e = 0; e < N; e++)
+      for (int v = 0; v < nv; v++) rhs[e * nv + v] = b[(size_t)c->iperm[e] * nv + v];
+  }
+  return BES_OK;
+}
+
+long bes_implicit_nnz(bes_ctx *c) {
+  if (!c) return -1;
+  long nz = 0;
+  for (int64_t e = 0; e < c->N; e++) {
+    nz++;
+    for (int m = 0; m < c->k; m++) if (c->geo.cadj[e * c->k + m] >= 0) nz++;
+  }
+  return nz * c->nv * c->nv;
+}
+
+int bes_linear_solve(bes_ctx *c, int *iters, double *relres) {
+  if (!c || !c->impl || !impl_of(c)->assembled) { set_error("system not assembled"); return BES_ERR_ARG; }
+  Impl *im = impl_of(c);
+  int s;
+  int it = 0;
+  double rr = 0;
+  if (im->prec == 64) {
+    if (!im->factored && (s = factor<double>(c, im))) return s;
+    im->factored = true;
+    s = gmres<double, double>(c, im, &it, &rr);
+  } else {
+    if (!im->factored && (s = factor<float>(c, im))) return s;
+    im->factored = true;
+    s = gmres<float, float>(c, im, &it, &rr);
+  }
+  if (s) return s;
+  if ((s = prof_collect(c))) return s;
+  im->last_iters = it;
+  im->last_relres = rr;
+  if (iters) *iters = it;
+  if (relres) *relres = rr;
+  if (!(rr <= im->rtol * 1.0001) && !(rr == 0)) {
+    set_error("linear solver stopped at relative residual " + std::to_string(rr));
+    return BES_ERR_SOLVE;
+  }
+  return BES_OK;
+}
+
+int bes_get_linear_solution(bes_ctx *c, double *x) {
+  if (!c || !c->impl || !x) { set_error("no solution"); return BES_ERR_ARG; }
+  Impl *im = impl_of(c);
+  const int N = c->N, nv = c->nv;
+  std::vector<double> h((size_t)N * nv);
+  BES_HIP(hipMemcpy(h.data(), im->x, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+  for (int64_t e = 0; e < N; e++)
+    for (int v = 0; v < nv; v++) x[e * nv + v] = h[(size_t)c->iperm[e] * nv + v];
+  return BES_OK;
+}
+
+int bes_implicit_update(bes_ctx *c) {
+  if (!c || !c->impl) { set_error("call bes_implicit_setup first"); return BES_ERR_ARG; }
+  Impl *im = impl_of(c);
+  const int bs = 256;
+  {
+    ProfScope ps(c, kProfUpdate, 24.0 * c->N * c->nv);
+    if (c->nd == 2) hipLaunchKernelGGL(k_update<2>, dim3(grid_for(c->N, bs)), dim3(bs), 0, c->stream, c->U0, im->x, c->N);
+    else hipLaunchKernelGGL(k_update<3>, dim3(grid_for(c->N, bs)), dim3(bs), 0, c->stream, c->U0, im->x, c->N);
+  }
+  BES_HIP(hipGetLastError());
+  BES_HIP(hipStreamSynchronize(c->stream));
+  return prof_collect(c);
+}
+
+// SolverImplicit::solve loop (SolverImplicit.cpp:552-600) with the adaptive
+// Courant number of SolverImplicit.cpp:530-550.  hist [nsteps][2*nv+3]:
+// Linf(b) per var, Linf(sum F*A) per var, CFL used, Krylov iterations, relres.
+int bes_implicit_run(bes_ctx *c, int nsteps, double cfl_min, double cfl_max, const double *exponents, int local,
+                     int repeat_nv, double stop_drop, double *hist, int *steps_done) {
+  if (!c || !c->impl || nsteps < 0 || !exponents) { set_error("invalid argument"); return BES_ERR_ARG; }
+  const int nv = c->nv;
+  double cfl = cfl_min;
+  std::vector<double> lb(nv), lr(nv), lb_old(nv, 0.0);
+  double r0 = -1;
+  int done = 0;
+  for (int it = 0; it < nsteps; it++) {
+    int s = bes_implicit_assemble(c, cfl, local, repeat_nv, lb.data(), lr.data());
+    if (s) return s;
+    int iters = 0;
+    double rr = 0;
+    s = bes_linear_solve(c, &iters, &rr);
+    if (s && s != BES_ERR_SOLVE) return s;
+    if ((s = bes_implicit_update(c))) return s;
+    if (hist) {
+      double *h = hist + (size_t)it * (2 * nv + 3);
+      for (int v = 0; v < nv; v++) { h[v] = lb[v]; h[nv + v] = lr[v]; }
+      h[2 * nv] = cfl;
+      h[2 * nv + 1] = iters;
+      h[2 * nv + 2] = rr;
+    }
+    done = it + 1;
+    if (r0 < 0) r0 = lr[0];
+    // adaptive Courant number (density monitor of the implicit residual)
+    if (cfl_max > cfl_min) {
+      const int ei = (lb_old[0] > lb[0]) ? 1 : 0;
+      cfl *= std::pow(lb_old[0] / lb[0], exponents[ei]);
+      cfl = std::max(cfl_min, cfl);
+      cfl = std::min(cfl_max, cfl);
+    }
+    lb_old = lb;
+    if (stop_drop > 0 && r0 > 0 && lr[0] <= stop_drop * r0) break;
+  }
+  if (steps_done) *steps_done = done;
+  return BES_OK;
+}
+
+}  // extern "C"

@@ -265,6 +265,246 @@ def tri_patch(ni=4, nj=3, seed=3, jitter=0.2, slip=("bottom",)):
 
 
 # --------------------------------------------------------------------------
+# 3D swept tapered wing in a half-span box
+# --------------------------------------------------------------------------
+
+NACA_CLOSED_TE = (0.2969, -0.1260, -0.3516, 0.2843, -0.1036)
+
+
+def naca_half_thickness(xi, t=0.10, coeff=NACA_CLOSED_TE):
+    """Half thickness of the 4-digit section per unit chord (closed trailing edge)."""
+    xi = np.clip(xi, 0.0, 1.0)
+    return 5.0 * t * (coeff[0] * np.sqrt(xi) + coeff[1] * xi + coeff[2] * xi ** 2
+                      + coeff[3] * xi ** 3 + coeff[4] * xi ** 4)
+
+
+@dataclasses.dataclass
+class WingSpec:
+    root_chord: float = 1.0
+    taper: float = 0.56
+    aspect_ratio: float = 3.8
+    sweep_le_deg: float = 30.0
+    thickness: float = 0.10
+    farfield: float = 10.0        # chords
+    tip_cap: float = 0.04         # fraction of the half span closed by an elliptic taper
+
+    @property
+    def half_span(self):
+        return self.aspect_ratio * self.root_chord * (1.0 + self.taper) / 4.0
+
+    @property
+    def area_half(self):
+        return 0.5 * self.half_span * self.root_chord * (1.0 + self.taper)
+
+    @property
+    def mac(self):
+        lam = self.taper
+        return 2.0 / 3.0 * self.root_chord * (1 + lam + lam * lam) / (1 + lam)
+
+
+def _stretch(n, h0, length):
+    """n cell sizes starting at h0 growing geometrically to cover length."""
+    if n <= 0:
+        return np.zeros(0)
+    lo, hi = 1.0, 2.0
+    for _ in range(200):
+        r = 0.5 * (lo + hi)
+        tot = h0 * (r ** n - 1) / (r - 1) if abs(r - 1) > 1e-12 else h0 * n
+        if tot > length:
+            hi = r
+        else:
+            lo = r
+    r = 0.5 * (lo + hi)
+    s = h0 * np.cumsum(r ** np.arange(n))
+    return s * (length / s[-1])
+
+
+def swept_wing(ni=96, nj=48, nk=72, seed=20261202, spec: WingSpec = None, jitter=0.08):
+    """Tetrahedral mesh around a swept tapered wing in a half-span box.
+
+    The lattice has ni x nj x nk hexahedra (6 tetrahedra each).  The wing is a
+    slit in the lattice: on the plane k = nk/2, nodes inside the planform
+    (chordwise 50% of the i range, spanwise ~60% of the j range) are
+    duplicated into an upper and a lower copy pushed onto the thickness
+    surface; the leading edge, trailing edge and tip lines stay shared, where
+    the thickness vanishes (closed trailing edge, elliptic tip taper).  Node
+    columns between the surface and the far field are compressed so every
+    hexahedron stays valid; cells cluster geometrically toward the wing.
+    Interior nodes away from the wing get a seeded jitter.  Boundaries: the
+    root plane y = 0 (symmetry) and the wing are slip walls, the outer box is
+    freestream.  x is chordwise, y spanwise, z up.
+    """
+    spec = spec or WingSpec()
+    rng = np.random.default_rng(seed)
+    c_r, s_half = spec.root_chord, spec.half_span
+    tan_sw = np.tan(np.radians(spec.sweep_le_deg))
+    ff = spec.farfield * c_r
+    iLE = ni // 4
+    iTE = iLE + ni // 2
+    jt = int(round(nj * 0.62))
+    kS = nk // 2
+    nchord = iTE - iLE
+
+    # spanwise stations: uniform-ish on the wing, clustered toward the tip,
+    # then geometric out to the far field
+    sj = np.linspace(0.0, 1.0, jt + 1)
+    y_wing = s_half * np.sin(0.5 * np.pi * sj) ** 1.0 * 0.5 + s_half * sj * 0.5
+    hy = y_wing[-1] - y_wing[-2]
+    y_out = s_half + _stretch(nj - jt, hy, ff)
+    Y = np.concatenate([y_wing, y_out])
+
+    def x_le(y):
+        return np.minimum(y, s_half) * tan_sw
+
+    def chord(y):
+        return c_r * (1.0 - (1.0 - spec.taper) * np.minimum(y, s_half) / s_half)
+
+    # chordwise parameter on the wing (cosine clustering at LE and TE)
+    xi = 0.5 * (1.0 - np.cos(np.pi * np.linspace(0.0, 1.0, nchord + 1)))
+    # vertical distribution from the slit plane
+    hz0 = 0.004 * c_r
+    z_up = _stretch(nk - kS, hz0, ff)
+    z_dn = _stretch(kS, hz0, ff)
+    Zt = np.concatenate([-z_dn[::-1], [0.0], z_up])   # nk+1 values, Zt[kS] = 0
+
+    NI, NJ, NK = ni + 1, nj + 1, nk + 1
+    X = np.zeros((NI, NJ))
+    Zs_up = np.zeros((NI, NJ))   # surface z of the upper copy (0 off the wing)
+    for j in range(NJ):
+        y = Y[j]
+        xl, c = x_le(y), chord(y)
+        xw = xl + c * xi
+        h_le = c * (xi[1] - xi[0]) * 4.0
+        h_te = c * (xi[-1] - xi[-2]) * 4.0
+        x_up = xl - _stretch(iLE, h_le, ff + xl)[::-1]
+        x_dn = xl + c + _stretch(ni - iTE, h_te, ff + c_r - (xl + c))
+        X[:, j] = np.concatenate([x_up, xw, x_dn])
+        if j <= jt:
+            yc = spec.tip_cap * s_half
+            tau = 1.0 if y <= s_half - yc else np.sqrt(max(0.0, 1.0 - ((y - (s_half - yc)) / yc) ** 2))
+            if j == jt:
+                tau = 0.0
+            Zs_up[iLE:iTE + 1, j] = c * naca_half_thickness(xi, spec.thickness) * tau
+    Zs_up[iLE, :] = 0.0
+    Zs_up[iTE, :] = 0.0
+
+    # node coordinates of the full lattice (upper copies on the slit plane)
+    P = np.zeros((NI, NJ, NK, 3))
+    P[..., 0] = X[:, :, None]
+    P[..., 1] = Y[None, :, None]
+    zs = Zs_up[:, :, None]
+    ztop, zbot = Zt[-1], -Zt[0]
+    zk = Zt[None, None, :]
+    upper = zk >= 0
+    P[..., 2] = np.where(upper, zs + zk * (1.0 - zs / ztop), -zs + zk * (1.0 - zs / zbot))
+    # seeded jitter of interior nodes away from the wing and the boundaries,
+    # in lattice-index space: a node moves by a fraction of the vector to its
+    # lattice neighbour along each axis, so sheared cells stay valid
+    J = rng.uniform(-jitter, jitter, size=P.shape)
+    kk = np.arange(NK)[None, None, :]
+    near = (np.abs(kk - kS) <= 2) & (np.arange(NI)[:, None, None] >= iLE - 2) & \
+           (np.arange(NI)[:, None, None] <= iTE + 2) & (np.arange(NJ)[None, :, None] <= jt + 2)
+    inner = np.zeros(P.shape[:3], dtype=bool)
+    inner[1:-1, 1:-1, 1:-1] = True
+    ii = np.arange(NI)[:, None, None]
+    thin = (np.abs(ii - iLE) <= 2) | (np.abs(ii - iTE) <= 2)   # sheared LE/TE columns
+    move = inner & ~near & ~thin
+    P0 = P.copy()
+    for a in range(3):
+        fwd = np.zeros_like(P0)
+        bwd = np.zeros_like(P0)
+        sl_lo = [slice(None)] * 3
+        sl_hi = [slice(None)] * 3
+        sl_lo[a] = slice(0, -1)
+        sl_hi[a] = slice(1, None)
+        fwd[tuple(sl_lo)] = P0[tuple(sl_hi)] - P0[tuple(sl_lo)]
+        bwd[tuple(sl_hi)] = P0[tuple(sl_hi)] - P0[tuple(sl_lo)]
+        t = J[..., a][..., None]
+        step = np.where(t > 0, t * fwd, t * bwd)
+        P += np.where(move[..., None], step, 0.0)
+
+    # node ids: lattice nodes, then the lower copies of interior slit nodes
+    base = np.arange(NI * NJ * NK).reshape(NI, NJ, NK)
+    slit = np.zeros((NI, NJ), dtype=bool)
+    slit[iLE + 1:iTE, 0:jt] = True          # interior of the planform (incl. root row j=0)
+    nlow = int(slit.sum())
+    low_id = np.full((NI, NJ), -1, dtype=np.int64)
+    low_id[slit] = NI * NJ * NK + np.arange(nlow)
+    lowP = P[:, :, kS, :][slit].copy()
+    lowP[:, 2] = -Zs_up[slit]
+    nodes = np.concatenate([P.reshape(-1, 3), lowP], axis=0)
+
+    def node_id(i, j, k, below):
+        nid = base[i, j, k]
+        if below is None:
+            return nid
+        use_low = below & (k == kS) & slit[i, j]
+        return np.where(use_low, low_id[i, j], nid)
+
+    # cells below the slit plane (hex k index = kS-1) use the lower copies
+    I, Jj, K = np.meshgrid(np.arange(ni), np.arange(nj), np.arange(nk), indexing="ij")
+    I = I.ravel(); Jj = Jj.ravel(); K = K.ravel()
+    below_hex = K == kS - 1
+    tets = []
+    signs = []
+    for perm in _KUHN:
+        p = np.stack([I, Jj, K], 1)
+        verts = [node_id(p[:, 0], p[:, 1], p[:, 2], below_hex)]
+        for ax in perm:
+            p = p.copy()
+            p[:, ax] += 1
+            verts.append(node_id(p[:, 0], p[:, 1], p[:, 2], below_hex))
+        tets.append(np.stack(verts, 1))
+        signs.append(1 if perm in [(0, 1, 2), (1, 2, 0), (2, 0, 1)] else -1)
+    tets = np.stack(tets, 1)   # [nhex, 6, 4]
+    vol6 = np.stack([tet_signed_volume6(nodes, tets[:, q]) for q in range(6)], 1)
+    expect = np.array(signs)[None, :]
+    bad = (vol6 * expect <= 0).sum()
+    if bad:
+        raise ValueError(f"swept_wing: {bad} inverted tetrahedra; coarsen or reduce jitter")
+    tets = tets.reshape(-1, 4)
+    tets = _orient_tets(nodes, tets)
+
+    bf = boundary_faces_of(tets, 3)
+    cen = nodes[bf].mean(axis=1)
+    eps = 1e-9
+    outer = ((np.abs(cen[:, 0] - nodes[:, 0].min()) < eps) | (np.abs(cen[:, 0] - nodes[:, 0].max()) < eps) |
+             (np.abs(cen[:, 1] - Y[-1]) < eps) | (np.abs(cen[:, 2] - Zt[0]) < eps) |
+             (np.abs(cen[:, 2] - Zt[-1]) < eps))
+    btype = np.where(outer, FREESTREAM, SLIP_WALL).astype(np.int32)
+    md = MeshData(3, nodes, tets.astype(np.int64), bf.astype(np.int64), btype)
+    md.wing = dict(spec=spec, iLE=iLE, iTE=iTE, jt=jt, kS=kS)
+    return md
+
+
+def wing_face_mask(md, faces_center, btype):
+    """Slip-wall faces on the wing (not the y = 0 symmetry plane)."""
+    return (btype == SLIP_WALL) & (np.abs(faces_center[:, 1]) > 1e-12)
+
+
+def force_coefficients(S, center, btype, owner, U, mach, alpha_deg, spec: WingSpec, gamma=1.4):
+    """Lift / drag / side-force coefficients of the wing from cell pressures.
+
+    Pressure force sum_f p_owner S_f over the wing's slip-wall faces (zeroth-
+    order face pressure, the value the reference writes to slip_wall_*.csv),
+    reference area = half-wing planform area, dynamic pressure 0.5*M^2
+    (non-dimensional rho=1, c=1).  Returns (CL, CD, Cp per face, mask).
+    """
+    m = wing_face_mask(None, center, btype)
+    Uc = U[owner[m]]
+    rho = Uc[:, 0]
+    mom = Uc[:, 1:4]
+    p = (gamma - 1.0) * (Uc[:, 4] - 0.5 * (mom * mom).sum(1) / rho)
+    F = (p[:, None] * S[m]).sum(0)
+    a = np.radians(alpha_deg)
+    q = 0.5 * mach * mach
+    lift = -F[0] * np.sin(a) + F[2] * np.cos(a)
+    drag = F[0] * np.cos(a) + F[2] * np.sin(a)
+    cp = (gamma * p - 1.0) / (0.5 * gamma * mach * mach)
+    return lift / (q * spec.area_half), drag / (q * spec.area_half), cp, m
+
+
+# --------------------------------------------------------------------------
 # MSH2 writer / reader (the reference's accepted subset)
 # --------------------------------------------------------------------------
 

@@ -119,15 +119,54 @@ int bes_flux_batch(int ndim, int kind, long n, const double *UL, const double *U
 int bes_jacobian_batch(int ndim, int kind, long n, const double *UL, const double *UR,
                        const double *normals, const bes_gas *gas, double *JL, double *JR);
 
+/* ---- implicit path: SolverImplicit + CoefficientMatrix + the AMGCL solve ----
+ * bes_implicit_setup: allocate the matrix and solver workspace.
+ *   precision 64 (FP64 matrix, factors and Krylov vectors) or 32 (FP32 inner
+ *   solve; the right-hand side and the update stay FP64); precond 0 none,
+ *   1 block Jacobi, 2 block ILU(0) in multicolour order; GMRES(restart),
+ *   restart <= 30, stops at ||b - A x|| <= rtol ||b|| (the reference's
+ *   LINEAR_SOLVER_ABS_TOL is this relative tolerance: CoefficientMatrix.cpp:304).
+ *   Replaces CoefficientMatrix::initialize (CoefficientMatrix.cpp:114-164). */
+int bes_implicit_setup(bes_ctx *ctx, int precision, int precond, int restart, int maxiter, double rtol);
+/* time step, gradient, limiter, face Jacobians and the system A x = b at the
+ * current state (SolverImplicit.cpp:307-320 = resetSystemOfEquation_ + dt +
+ * gradient + limiter + computeFaceFluxes_).  repeat_nv != 0 reproduces the
+ * reference's nV-fold addition of every Jacobian block (SolverImplicit.cpp:
+ * 425-502).  linf_b [nv] = Linf of b (the reference's implicit monitor,
+ * SolverImplicit.cpp:520-523), linf_r [nv] = Linf of sum_f +-F*A; may be NULL. */
+int bes_implicit_assemble(bes_ctx *ctx, double cfl, int local, int repeat_nv, double *linf_b, double *linf_r);
+/* the assembled system in the reference's CoefficientMatrix CSR layout
+ * (CoefficientMatrix.cpp:127-160; MatrixSparse.hpp getRowPtr/getColPtr/getValPtr)
+ * and right-hand side [ncell][nv]; nnz = bes_implicit_nnz(ctx) */
+long bes_implicit_nnz(bes_ctx *ctx);
+int bes_implicit_get_system(bes_ctx *ctx, long *rowptr, long *col, double *val, double *rhs);
+/* CoefficientMatrix::solve (CoefficientMatrix.cpp:210-325): iterations and the
+ * true relative residual ||b - A x|| / ||b||; BES_ERR_SOLVE if rtol was not met */
+int bes_linear_solve(bes_ctx *ctx, int *iters, double *relres);
+int bes_get_linear_solution(bes_ctx *ctx, double *x /*[ncell][nv]*/);
+/* U += x (SolverImplicit::updateSolution_, SolverImplicit.cpp:509-528) */
+int bes_implicit_update(bes_ctx *ctx);
+/* SolverImplicit::solve (SolverImplicit.cpp:552-600): up to nsteps
+ * backward-Euler steps with the adaptive Courant number of SolverImplicit.cpp:
+ * 530-550 (cfl_min = COURANT_NUM, cfl_max = ADAPTIVE_COURANT_NUM_MAX,
+ * exponents = ADAPTIVE_COURANT_NUM_EXPONENT).  Stops early when the Linf of
+ * the density residual sum_f +-F*A falls below stop_drop times its first value
+ * (stop_drop <= 0: never).  hist [nsteps][2*nv+3] (may be NULL): Linf(b) per
+ * var, Linf(sum F*A) per var, Courant number, Krylov iterations, relres. */
+int bes_implicit_run(bes_ctx *ctx, int nsteps, double cfl_min, double cfl_max, const double *exponents,
+                     int local, int repeat_nv, double stop_drop, double *hist, int *steps_done);
+
 /* ---- measurement (no counterpart in the reference, which times nothing):
  *      when profiling is on, every kernel launch is bracketed by HIP events on
  *      the context's stream; bes_get_profile returns, per kernel kind, the
- *      summed device time and the launch count since profiling was switched on.
+ *      summed device time, the launch count and the summed algorithmic HBM
+ *      bytes (every array a launch must touch, counted once; neighbour
+ *      gathers served by cache not counted) since profiling was switched on.
  *      kinds: 0 time step, 1 gradient+limiter, 2 residual+update,
  *      3 Jacobian, 4 assembly, 5 SpMV, 6 preconditioner apply, 7 Krylov vector
  *      ops, 8 factorisation, 9 implicit update ---- */
 int bes_set_profiling(bes_ctx *ctx, int on);
-int bes_get_profile(bes_ctx *ctx, double *total_ms, long *launches, int nkinds);
+int bes_get_profile(bes_ctx *ctx, double *total_ms, long *launches, double *bytes, int nkinds);
 int bes_synchronize(bes_ctx *ctx);
 
 /* ---- error reporting ---- */
