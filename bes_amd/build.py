@@ -16,6 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libbes_hip.so")
+CLI = os.path.join(HERE, "bes_hip")
 SOURCES = ["explicit.hip", "implicit.hip", "geometry.cpp"]
 HEADERS = ["context.hpp", "flux.hpp", "geometry.hpp", "kernels_common.hpp"]
 
@@ -37,6 +38,17 @@ def build(verbose: bool = False) -> str:
         sys.stderr.write(res.stdout + res.stderr)
         raise RuntimeError("hipcc failed building libbes_hip.so")
     os.replace(LIB + ".tmp", LIB)
+    # command-line driver over the C++ classes of include/bes/bes.hpp
+    cli = [os.environ.get("CXX", "g++"), "-std=c++17", "-O2", "-Wall", "-I" + os.path.join(ROOT, "include"),
+           "-o", CLI + ".tmp", os.path.join(CSRC, "bes_main.cpp"), "-L" + HERE, "-lbes_hip",
+           "-L/opt/rocm/lib", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,/opt/rocm/lib"]
+    if verbose:
+        print(" ".join(cli), flush=True)
+    res = subprocess.run(cli, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError("building the bes_hip driver failed")
+    os.replace(CLI + ".tmp", CLI)
     return LIB
 
 

@@ -1135,6 +1135,39 @@ int bes_implicit_get_system(bes_ctx *c, long *rowptr, long *col, double *val, do
   return BES_OK;
 }
 
+// load a system assembled on the host in the reference CSR layout (the
+// CoefficientMatrix::addToBlock / addToVal path, CoefficientMatrix.cpp:166-204)
+int bes_implicit_set_system(bes_ctx *c, const double *val, const double *rhs) {
+  if (!c || !c->impl || !val || !rhs) { set_error("call bes_implicit_setup first / null argument"); return BES_ERR_ARG; }
+  Impl *im = impl_of(c);
+  const int N = c->N, nv = c->nv, k = c->k, NB = nv * nv;
+  const size_t mat = (size_t)(k + 1) * NB * N;
+  std::vector<double> A(mat, 0.0), b((size_t)N * nv);
+  const Geometry &g = c->geo;
+  long nz = 0;
+  for (int64_t e = 0; e < N; e++) {
+    const int i = (int)c->iperm[e];
+    for (int a = 0; a < nv; a++) {
+      for (int bq = 0; bq < nv; bq++) A[((size_t)a * nv + bq) * N + i] = val[nz++];
+      for (int m = 0; m < k; m++) {
+        if (g.cadj[e * k + m] < 0) continue;
+        for (int bq = 0; bq < nv; bq++) A[((size_t)(1 + m) * NB + a * nv + bq) * N + i] = val[nz++];
+      }
+    }
+    for (int v = 0; v < nv; v++) b[(size_t)i * nv + v] = rhs[e * nv + v];
+  }
+  if (im->prec == 64) {
+    BES_HIP(hipMemcpy(im->A, A.data(), mat * sizeof(double), hipMemcpyHostToDevice));
+  } else {
+    std::vector<float> Af(A.begin(), A.end());
+    BES_HIP(hipMemcpy(im->A, Af.data(), mat * sizeof(float), hipMemcpyHostToDevice));
+  }
+  BES_HIP(hipMemcpy(im->b, b.data(), b.size() * sizeof(double), hipMemcpyHostToDevice));
+  im->assembled = true;
+  im->factored = false;
+  return BES_OK;
+}
+
 long bes_implicit_nnz(bes_ctx *c) {
   if (!c) return -1;
   long nz = 0;

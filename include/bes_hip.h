@@ -140,6 +140,10 @@ int bes_implicit_assemble(bes_ctx *ctx, double cfl, int local, int repeat_nv, do
  * and right-hand side [ncell][nv]; nnz = bes_implicit_nnz(ctx) */
 long bes_implicit_nnz(bes_ctx *ctx);
 int bes_implicit_get_system(bes_ctx *ctx, long *rowptr, long *col, double *val, double *rhs);
+/* load a system assembled by the caller in the same layout (the host-side
+ * CoefficientMatrix::changeAllVal/addToDiagonal/addToVal/addToBlock path,
+ * CoefficientMatrix.cpp:166-204); val [nnz], rhs [ncell][nv] */
+int bes_implicit_set_system(bes_ctx *ctx, const double *val, const double *rhs);
 /* CoefficientMatrix::solve (CoefficientMatrix.cpp:210-325): iterations and the
  * true relative residual ||b - A x|| / ||b||; BES_ERR_SOLVE if rtol was not met */
 int bes_linear_solve(bes_ctx *ctx, int *iters, double *relres);

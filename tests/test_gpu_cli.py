@@ -1,0 +1,81 @@
+"""The bes_hip driver (reference-style CLI over the C++ classes) end to end.
+
+Writes a reference-format .besi and MSH2 mesh, runs ``bes_amd/bes_hip`` and
+compares residual.csv (the reference's format, SolverExplicit.cpp:380-390,
+SolverImplicit.cpp:328-343) with the oracle run on the same mesh.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+from bes_amd import meshgen as mg
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+CLI = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "bes_amd", "bes_hip")
+
+
+def _run(tmp_path, md, solver, nsteps, extra=()):
+    mg.write_msh2(str(tmp_path / "m.msh"), md)
+    mg.write_besi(str(tmp_path / "c.besi"), "m.msh", md.ndim, mach=(0.5, 0.05, 0.0), solver=solver,
+                  cfl=0.8 if solver == "EXPLICIT" else 5.0, cfl_max=0.0 if solver == "EXPLICIT" else 50.0,
+                  exponents=(1.1, 0.75), nsteps=nsteps, output=".")
+    if extra:
+        with open(tmp_path / "c.besi", "a") as fh:
+            fh.write("\n".join(extra) + "\n")
+    res = subprocess.run([CLI, "c.besi"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    return res
+
+
+def _csv(path):
+    rows = [l.rstrip(",\n").split(",") for l in open(path) if l.strip()]
+    return np.array([[float(x) for x in r] for r in rows])
+
+
+def test_cli_explicit_matches_oracle(tmp_path):
+    md = mg.bump_channel(30, 15, seed=9)
+    res = _run(tmp_path, md, "EXPLICIT", 6)
+    assert res.returncode == 0, res.stderr
+    got = _csv(tmp_path / "residual.csv")
+    finf = orc.freestream_primitive((0.5, 0.05), 2)
+    U = np.tile(orc.conserved_from_primitive(finf, 2), (md.ncell, 1))
+    _, linf = orc.Mesh(md).explicit(U, 6, 0.8, finf, nthreads=1)
+    assert got.shape == (6, 5)
+    assert np.array_equal(got[:, 0], np.arange(1, 7))
+    np.testing.assert_allclose(got[:, 1:], linf, rtol=6e-6, atol=1e-300)
+
+
+def test_cli_implicit_matches_oracle(tmp_path):
+    md = mg.bump_channel(24, 12, seed=10)
+    res = _run(tmp_path, md, "IMPLICIT", 5, extra=["LINEAR_SOLVER_ABS_TOL = 1E-12",
+                                                   "LINEAR_SOLVER_MAX_ITER = 2000"])
+    assert res.returncode == 0, res.stderr
+    got = _csv(tmp_path / "residual.csv")
+    # oracle: reference implicit loop with a direct solve and the adaptive CFL
+    finf = orc.freestream_primitive((0.5, 0.05), 2)
+    om = orc.Mesh(md)
+    U = np.tile(orc.conserved_from_primitive(finf, 2), (md.ncell, 1))
+    cfl, old, ref = 5.0, np.zeros(4), []
+    for _ in range(5):
+        rp, col, val, rhs, _ = om.implicit_assemble(U, cfl, finf)
+        A = sp.bsr_matrix((val, col, rp), shape=(md.ncell * 4,) * 2).tocsc()
+        U = U + spla.spsolve(A, rhs.ravel()).reshape(U.shape)
+        lb = np.abs(rhs).max(0)
+        ref.append(lb)
+        e = 0.75 if old[0] > lb[0] else 1.1   # exponent[1] when the residual fell (:537)
+        cfl = min(50.0, max(5.0, cfl * (old[0] / lb[0]) ** e))
+        old = lb
+    assert got.shape == (5, 1 + 4 + 2)
+    np.testing.assert_allclose(got[:, 1:5], np.array(ref), rtol=1e-5)
+    assert (got[:, 6] <= 1e-12 * 1.0001).all()
+
+
+def test_cli_rejects_unknown_key(tmp_path):
+    md = mg.tri_patch(3, 3)
+    res = _run(tmp_path, md, "EXPLICIT", 1, extra=["NOT_A_KEY = 3"])
+    assert res.returncode == 1
+    assert "unknown config key" in res.stderr
