@@ -79,7 +79,7 @@ OPTIONAL_SIGS = {
     "bes_synchronize": [C.c_void_p],
 }
 PROF_KINDS = ["timestep", "gradlim", "residual", "jacobian", "assemble", "spmv", "precond",
-              "krylov", "factor", "update"]
+              "krylov", "factor", "update", "faceflux"]
 
 
 def lib():
@@ -149,6 +149,11 @@ class Context:
         self.ndim, self.ncell, self.nface = nd.value, nc.value, nf.value
         self.nv = self.ndim + 2
         self.k = self.ndim + 1
+        self.batch = 1
+
+    def _shape(self, *tail):
+        # per-condition arrays carry a leading batch axis when batch > 1
+        return (self.batch,) + tail if self.batch > 1 else tail
 
     def close(self):
         if getattr(self, "h", None):
@@ -180,32 +185,46 @@ class Context:
         return o
 
     # ---- state ----
+    def set_batch(self, nbatch):
+        """Advance nbatch freestream conditions (a power of two <= 64) together.
+
+        Per-condition arrays (state, dt, gradient, limiter, residual, monitors)
+        then carry a leading axis of length nbatch; the device state is discarded.
+        """
+        _check(lib().bes_set_batch(self.h, int(nbatch)))
+        self.batch = int(nbatch)
+
     def set_freestream(self, primitive):
+        """Freestream primitive [nv+1] for every condition, or [batch][nv+1]."""
         p = np.ascontiguousarray(primitive, np.float64)
+        if p.ndim == 2:
+            assert p.shape == (self.batch, self.nv + 1)
+            _check(lib().bes_set_freestream_batch(self.h, _p(p)))
+            return
         assert p.shape == (self.nv + 1,)
         _check(lib().bes_set_freestream(self.h, _p(p)))
 
     def set_state(self, U):
         U = np.ascontiguousarray(U, np.float64)
-        assert U.shape == (self.ncell, self.nv)
+        assert U.shape == self._shape(self.ncell, self.nv)
         _check(lib().bes_set_state(self.h, _p(U)))
 
     def get_state(self):
-        U = np.zeros((self.ncell, self.nv))
+        U = np.zeros(self._shape(self.ncell, self.nv))
         _check(lib().bes_get_state(self.h, _p(U)))
         return U
 
     # ---- passes ----
     def time_step(self, cfl, local=True):
         _check(lib().bes_compute_time_step(self.h, cfl, int(local)))
-        dt = np.zeros(self.ncell)
+        dt = np.zeros(self._shape(self.ncell))
         _check(lib().bes_get_time_step(self.h, _p(dt)))
         return dt
 
     def gradient_limiter(self):
         _check(lib().bes_compute_gradient_limiter(self.h))
-        G = np.zeros((self.ncell, self.nv, self.ndim))
-        P = np.zeros((self.ncell, self.nv))
+        G = np.zeros(self._shape(self.ncell, self.nv, self.ndim))
+        P = np.zeros(self._shape(self.ncell, self.nv))
         _check(lib().bes_get_gradient(self.h, _p(G)))
         _check(lib().bes_get_limiter(self.h, _p(P)))
         return G, P
@@ -213,8 +232,8 @@ class Context:
     def residual(self):
         """Second-order residual of the current state (gradient/limiter recomputed)."""
         _check(lib().bes_compute_gradient_limiter(self.h))
-        R = np.zeros((self.ncell, self.nv))
-        mx = np.zeros(self.nv)
+        R = np.zeros(self._shape(self.ncell, self.nv))
+        mx = np.zeros(self._shape(self.nv))
         _check(lib().bes_compute_residual(self.h, _p(R), _p(mx)), self.h)
         return R, mx
 
@@ -282,9 +301,13 @@ class Context:
                                       stop_drop, _p(hist), C.byref(done)), self.h)
         return hist[:done.value]
 
+    def explicit_graph(self, on=True):
+        """Replay each pseudo-time step as one captured HIP graph."""
+        _check(lib().bes_explicit_graph(self.h, int(on)))
+
     def explicit_run(self, niter, cfl, local=True, alpha=RK4_ALPHA, monitor=True):
         a = np.ascontiguousarray(alpha, np.float64)
-        linf = np.zeros((niter, self.nv)) if monitor else None
+        linf = np.zeros((niter,) + self._shape(self.nv)) if monitor else None
         _check(lib().bes_explicit_run(self.h, niter, len(a), _p(a), cfl, int(local), _p(linf)), self.h)
         return linf
 

@@ -25,9 +25,15 @@ struct DevMesh {
   double *rl = nullptr;     // [k][nd][N] centroid -> face centre
   double *nf = nullptr;     // [k][nd][N] face unit normal, owner -> neighbour
   double *af = nullptr;     // [k][N] face area
-  double *rs = nullptr;     // [k][nd][N] reconstruction vector of this cell
-  double *rn = nullptr;     // [k][nd][N] reconstruction vector of the neighbour
+  int *cfd = nullptr;       // [k][N] device face index of (cell, local face)
   uint32_t *info = nullptr; // [N] bits 0-7: face order (2 bits each); 8-11: owner flags
+  // faces in device order: sorted by the owner's (Morton) index, then local face
+  int F = 0;
+  int *fo = nullptr;        // [F] owner cell
+  int *fnb = nullptr;       // [F] neighbour cell, -1 slip wall, -2 far field
+  double *fnr = nullptr;    // [nd][F] unit normal, owner -> neighbour
+  double *frs = nullptr;    // [nd][F] owner's reconstruction vector for the face
+  double *frn = nullptr;    // [nd][F] neighbour's reconstruction vector for the face
 };
 
 // Matrix storage of the implicit path (block CSR, own block first in each row).
@@ -47,21 +53,39 @@ struct bes_ctx {
   int nd = 0, nv = 0, k = 0, N = 0;
   bes::GasConst gc{};
   double venkat_k = 5.0;
-  double finf[6] = {0, 0, 0, 0, 0, 0};
+  // B freestream conditions share the mesh (B = 1 << lgB <= kMaxBatch); every
+  // per-cell device array below carries the condition as its fastest index
+  static constexpr int kMaxBatch = 64;
+  int B = 1, lgB = 0;
+  double finf[6] = {0, 0, 0, 0, 0, 0};  // freestream primitive of condition 0
+  std::vector<double> finf_b;           // [B][nv+1] freestream primitive per condition
+  double *dfinf = nullptr;              // device copy of finf_b
   bes::DevMesh dm;
   hipStream_t stream = nullptr;
-  double *U0 = nullptr, *Ua = nullptr, *Ub = nullptr;  // [nv][N]
-  double *LG = nullptr;        // [nv][nd][N] limited gradient phi*grad
-  double *G = nullptr;         // [nv][nd][N] gradient (optional output)
-  double *phi = nullptr;       // [nv][N] limiter (optional output)
-  double *dt = nullptr;        // [N]
-  double *dtg = nullptr;       // [1] global dt
+  double *U0 = nullptr, *Ua = nullptr, *Ub = nullptr;  // [nv][N][B]
+  double *LG = nullptr;        // [N][nv*(1+nd)][B] per-cell pack: U, then phi*grad [v][d]
+  double *FF = nullptr;        // [F][nv][B] face fluxes per unit area (device face order)
+  std::vector<int64_t> fref;   // reference face id of each device face
+  double *G = nullptr;         // [nv][nd][N][B] gradient (optional output)
+  double *phi = nullptr;       // [nv][N][B] limiter (optional output)
+  double *dt = nullptr;        // [N][B]
+  double *dtg = nullptr;       // [B] global dt
   bool dt_local = true;        // mode of the last time-step computation
-  double *R = nullptr;         // [nv][N] residual (optional output)
-  unsigned long long *red = nullptr;   // reduction scratch, 64 words
-  unsigned long long *err = nullptr;   // first failing (cell*k + m), ~0 if none
-  double *linf = nullptr;              // [cap][nv] per-iteration Linf
+  double *R = nullptr;         // [nv][N][B] residual (optional output)
+  // reduction scratch: [0, 64) global-dt minimum per condition, [64, 64 + nv*B)
+  // Linf bit patterns [v][B]
+  static constexpr int kRedWords = 64 + 5 * kMaxBatch;
+  unsigned long long *red = nullptr;
+  unsigned long long *err = nullptr;   // first failing device face, ~0 if none
+  double *linf = nullptr;              // [cap][B][nv] per-iteration Linf
   int linf_cap = 0;
+  // pseudo-time step captured as a HIP graph (bes_explicit_graph)
+  bool use_graph = false;
+  hipGraphExec_t graph = nullptr;
+  int graph_key[4] = {0, 0, 0, 0};     // nstage, local, linf on, B of the capture
+  double graph_cfl = 0.0;
+  std::vector<double> graph_alpha;
+  int *dit = nullptr;                  // device iteration counter of graph replays
   // implicit path
   bes::DevBsr bsr;
   void *impl = nullptr;        // implicit solver workspace (implicit.hip)
@@ -89,7 +113,7 @@ inline int grid_for(int n, int bs) { return (n + bs - 1) / bs; }
 // kernel kinds for per-kernel timing
 enum ProfKind { kProfTimestep = 0, kProfGradlim = 1, kProfResidual = 2, kProfJacobian = 3,
                 kProfAssemble = 4, kProfSpmv = 5, kProfPrecond = 6, kProfKrylov = 7, kProfFactor = 8,
-                kProfUpdate = 9 };
+                kProfUpdate = 9, kProfFaceFlux = 10 };
 // bracket a launch with events when profiling is on
 // bytes: the launch's algorithmic HBM bytes (every array it must touch, once)
 struct ProfScope {

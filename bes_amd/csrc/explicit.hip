@@ -80,35 +80,42 @@ int prof_collect(bes_ctx *c) {
 // kernels
 // ---------------------------------------------------------------------------
 
+// Batched layout: B freestream conditions share the mesh (B a power of two
+// <= 64, lgB = log2 B).  A state value of variable v, cell i, condition b sits
+// at ((v*N + i)*B + b): for B = 64 a wave holds one cell's 64 conditions, so
+// geometry loads are wave-uniform and state loads are coalesced 512-B rows;
+// for B = 1 the layout is the plain SoA U[v][N].
+
 template <int ND>
-__global__ __launch_bounds__(256) void k_timestep(const double *__restrict__ U, DevMesh dm,
-                                                  GasConst gc, double cfl,
-                                                  double *__restrict__ dt,
-                                                  unsigned long long *__restrict__ gmin) {
+__global__ __launch_bounds__(256) void k_timestep(const double *__restrict__ U, DevMesh dm, GasConst gc,
+                                                  double cfl, double *__restrict__ dt,
+                                                  unsigned long long *__restrict__ gmin, int lgB) {
   constexpr int NV = ND + 2;
-  const int N = dm.N;
-  const int i = cell_index();
-  double t = __builtin_inf();
+  const int N = dm.N, B = 1 << lgB;
+  const int t = cell_index();
+  const int i = t >> lgB, b = t & (B - 1);
+  double tv = __builtin_inf();
   if (i < N) {
     double u[NV];
 #pragma unroll
-    for (int v = 0; v < NV; v++) u[v] = U[(size_t)v * N + i];
+    for (int v = 0; v < NV; v++) u[v] = U[((size_t)v * N + i) * B + b];
     Gas<ND, double> g(gc);
     double c = g.sound_speed(u);
     double sr = 0.0;
 #pragma unroll
     for (int d = 0; d < ND; d++) sr += (c + ::fabs(g.velocity(u, d))) * dm.sproj[(size_t)d * N + i];
-    t = cfl * dm.vol[i] / sr;
-    dt[i] = t;
+    tv = cfl * dm.vol[i] / sr;
+    dt[(size_t)i * B + b] = tv;
   }
   if (gmin) {
-    double m = block_min(t);
-    if (threadIdx.x == 0) atomicMin(gmin, (unsigned long long)__double_as_longlong(m));
+    double m = block_min_cond(tv, B);
+    if ((int)threadIdx.x < B) atomicMin(&gmin[threadIdx.x], (unsigned long long)__double_as_longlong(m));
   }
 }
 
-__global__ void k_decode_min(const unsigned long long *bits, double *out) {
-  out[0] = __longlong_as_double((long long)bits[0]);
+__global__ void k_decode_min(const unsigned long long *bits, double *out, int n) {
+  const int q = threadIdx.x;
+  if (q < n) out[q] = __longlong_as_double((long long)bits[q]);
 }
 
 // reference LimiterVenkatakrishnan.cpp:101-128
@@ -122,19 +129,22 @@ __device__ __forceinline__ double face_limiter(double mn, double mx, double d2, 
   return 1.0;
 }
 
+// Green-Gauss gradient (GradientGreenGauss.cpp:21-135) fused with the
+// Venkatakrishnan limiter (LimiterVenkatakrishnan.cpp:21-99); writes the cell
+// pack [U | phi*grad] at ((i*PW + q)*B + b), PW = nv*(1+nd)
 template <int ND>
 __global__ __launch_bounds__(256) void k_gradlim(const double *__restrict__ U, DevMesh dm,
-                                                 double *__restrict__ LG,
-                                                 double *__restrict__ Gout,
-                                                 double *__restrict__ Pout) {
-  constexpr int NV = ND + 2, K = ND + 1;
-  const int N = dm.N;
-  const int i = cell_index();
+                                                 double *__restrict__ PK, double *__restrict__ Gout,
+                                                 double *__restrict__ Pout, int lgB) {
+  constexpr int NV = ND + 2, K = ND + 1, PW = NV * (1 + ND);
+  const int N = dm.N, B = 1 << lgB;
+  const int t = cell_index();
+  const int i = t >> lgB, b = t & (B - 1);
   if (i >= N) return;
   double u[NV], mn[NV], mx[NV], g[NV][ND];
 #pragma unroll
   for (int v = 0; v < NV; v++) {
-    u[v] = U[(size_t)v * N + i];
+    u[v] = U[((size_t)v * N + i) * B + b];
     mn[v] = u[v];
     mx[v] = u[v];
 #pragma unroll
@@ -150,7 +160,7 @@ __global__ __launch_bounds__(256) void k_gradlim(const double *__restrict__ U, D
       const double w = dm.w[(size_t)m * N + i];
 #pragma unroll
       for (int v = 0; v < NV; v++) {
-        const double uj = U[(size_t)v * N + j];
+        const double uj = U[((size_t)v * N + j) * B + b];
         const double fv = w * u[v] + (1 - w) * uj;
 #pragma unroll
         for (int d = 0; d < ND; d++) g[v][d] += S[d] * fv;
@@ -163,17 +173,17 @@ __global__ __launch_bounds__(256) void k_gradlim(const double *__restrict__ U, D
 #pragma unroll
         for (int d = 0; d < ND; d++) g[v][d] += S[d] * (u[v]);
       if (j == -1) {  // slip wall: tangential momentum on the face
-        double nn[ND], t[ND], vn = 0;
+        double nn[ND], tg[ND], vn = 0;
 #pragma unroll
         for (int d = 0; d < ND; d++) nn[d] = dm.nf[((size_t)m * ND + d) * N + i];
 #pragma unroll
         for (int d = 0; d < ND; d++) vn += u[1 + d] * nn[d];
 #pragma unroll
-        for (int d = 0; d < ND; d++) t[d] = u[1 + d] - vn * nn[d];
+        for (int d = 0; d < ND; d++) tg[d] = u[1 + d] - vn * nn[d];
 #pragma unroll
         for (int a = 0; a < ND; a++)
 #pragma unroll
-          for (int d = 0; d < ND; d++) g[1 + a][d] += (t[a] * S[d] - u[1 + a] * S[d]);
+          for (int d = 0; d < ND; d++) g[1 + a][d] += (tg[a] * S[d] - u[1 + a] * S[d]);
       }
     }
   }
@@ -199,100 +209,148 @@ __global__ __launch_bounds__(256) void k_gradlim(const double *__restrict__ U, D
     }
 #pragma unroll
     for (int d = 0; d < ND; d++) {
-      LG[((size_t)v * ND + d) * N + i] = best * g[v][d];
-      if (Gout) Gout[((size_t)v * ND + d) * N + i] = g[v][d];
+      if (Gout) Gout[(((size_t)v * ND + d) * N + i) * B + b] = g[v][d];
+      g[v][d] = best * g[v][d];   // phi*grad: the product the reconstruction multiplies
     }
-    if (Pout) Pout[(size_t)v * N + i] = best;
+    if (Pout) Pout[((size_t)v * N + i) * B + b] = best;
+  }
+  double pk[PW];
+#pragma unroll
+  for (int v = 0; v < NV; v++) {
+    pk[v] = u[v];
+#pragma unroll
+    for (int d = 0; d < ND; d++) pk[NV + v * ND + d] = g[v][d];
+  }
+  if (B == 1) {   // one contiguous run per cell: 16-byte stores
+    double2 *dst = reinterpret_cast<double2 *>(PK + (size_t)i * PW);
+#pragma unroll
+    for (int q = 0; q < PW / 2; q++) dst[q] = make_double2(pk[2 * q], pk[2 * q + 1]);
+  } else {        // condition-fastest: each store a coalesced row across the wave
+#pragma unroll
+    for (int q = 0; q < PW; q++) PK[((size_t)i * PW + q) * B + b] = pk[q];
   }
 }
 
-struct Finf {
-  double p[6];
-};
-
+// reconstructed face state U + (phi*grad).r of one side, from a cell pack
 template <int ND>
-__global__ __launch_bounds__(256) void k_residual(
+__device__ __forceinline__ void face_state(const double *__restrict__ PK, int cell, int b, int B,
+                                           const double *r, double *uf) {
+  constexpr int NV = ND + 2, PW = NV * (1 + ND);
+  double pk[PW];
+  if (B == 1) {
+    const double2 *src = reinterpret_cast<const double2 *>(PK + (size_t)cell * PW);
+#pragma unroll
+    for (int q = 0; q < PW / 2; q++) {
+      const double2 t = src[q];
+      pk[2 * q] = t.x;
+      pk[2 * q + 1] = t.y;
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < PW; q++) pk[q] = PK[((size_t)cell * PW + q) * B + b];
+  }
+#pragma unroll
+  for (int v = 0; v < NV; v++) {
+    uf[v] = pk[v];
+#pragma unroll
+    for (int d = 0; d < ND; d++) uf[v] += pk[NV + v * ND + d] * r[d];
+  }
+}
+
+// One thread per (face, condition), faces in device order: second-order states
+// of both sides from the cell packs, then the face flux per unit area
+// (FluxRoe.tpp:306-333, FluxFreestream.tpp:192-226, FluxSlipWall.tpp:32-53),
+// computed once per face.  finf: freestream primitive state per condition.
+template <int ND>
+__global__ __launch_bounds__(256) void k_face_flux(const double *__restrict__ PK, DevMesh dm, GasConst gc,
+                                                   const double *__restrict__ finf, double *__restrict__ FF,
+                                                   unsigned long long *__restrict__ err, int lgB) {
+  constexpr int NV = ND + 2;
+  const int F = dm.F, B = 1 << lgB;
+  const int t = xcd_block() * blockDim.x + threadIdx.x;
+  const int f = t >> lgB, b = t & (B - 1);
+  if (f >= F) return;
+  const int o = dm.fo[f], j = dm.fnb[f];
+  double nn[ND], rs[ND], ul[NV], out[NV];
+#pragma unroll
+  for (int d = 0; d < ND; d++) {
+    nn[d] = dm.fnr[(size_t)d * F + f];
+    rs[d] = dm.frs[(size_t)d * F + f];
+  }
+  face_state<ND>(PK, o, b, B, rs, ul);
+  if (j >= 0) {
+    double rn[ND], ur[NV];
+#pragma unroll
+    for (int d = 0; d < ND; d++) rn[d] = dm.frn[(size_t)d * F + f];
+    face_state<ND>(PK, j, b, B, rn, ur);
+    roe_flux<ND, double>(gc, ul, ur, nn, out);
+  } else if (j == -1) {
+    slip_flux<ND, double>(gc, ul, nn, out);
+  } else {
+    double fi[NV + 1];
+#pragma unroll
+    for (int q = 0; q < NV + 1; q++) fi[q] = finf[b * (NV + 1) + q];
+    freestream_flux<ND, double>(gc, ul, fi, nn, out);
+  }
+  bool bad = false;
+#pragma unroll
+  for (int v = 0; v < NV; v++) bad |= (out[v] != out[v]);
+  if (bad) atomicMin(err, (unsigned long long)f);
+#pragma unroll
+  for (int v = 0; v < NV; v++) FF[((size_t)f * NV + v) * B + b] = out[v];
+}
+
+// One thread per (cell, condition): residual sum_f +-F*A over the cell's faces
+// in the reference face order (SolverExplicit.cpp:325-341), fused with the
+// stage update U = U0 - alpha*dt/V*R (:364-375) and the Linf monitor (:356-362).
+template <int ND>
+__global__ __launch_bounds__(256) void k_cell_residual(
     // U0 and Ud may be the same array (the last stage updates in place: each
-    // thread reads only its own U0[i] before writing Ud[i]), so no restrict
-    const double *__restrict__ Us, const double *__restrict__ LG, const double *U0,
-    double *Ud, const double *__restrict__ dt, const double *__restrict__ dtg,
-    double alpha, DevMesh dm, GasConst gc, Finf fi, double *__restrict__ Rout,
-    unsigned long long *__restrict__ linf_bits, unsigned long long *__restrict__ err) {
+    // thread reads only its own U0 entry before writing it), so no restrict
+    const double *__restrict__ FF, const double *U0, double *Ud, const double *__restrict__ dt,
+    const double *__restrict__ dtg, double alpha, DevMesh dm, double *__restrict__ Rout,
+    unsigned long long *__restrict__ linf_bits, int lgB) {
   constexpr int NV = ND + 2, K = ND + 1;
-  const int N = dm.N;
-  const int i = cell_index();
+  const int N = dm.N, B = 1 << lgB;
+  const int t = cell_index();
+  const int i = t >> lgB, b = t & (B - 1);
   double R[NV];
 #pragma unroll
   for (int v = 0; v < NV; v++) R[v] = 0.0;
   if (i < N) {
-    double u[NV], lg[NV][ND];
-#pragma unroll
-    for (int v = 0; v < NV; v++) {
-      u[v] = Us[(size_t)v * N + i];
-#pragma unroll
-      for (int d = 0; d < ND; d++) lg[v][d] = LG[((size_t)v * ND + d) * N + i];
-    }
     const uint32_t info = dm.info[i];
+#pragma unroll
     for (int q = 0; q < K; q++) {
       const int m = (info >> (2 * q)) & 3;
       const bool own = (info >> (8 + m)) & 1;
-      const int j = dm.nbr[(size_t)m * N + i];
-      double nn[ND], rs[ND];
-#pragma unroll
-      for (int d = 0; d < ND; d++) {
-        nn[d] = dm.nf[((size_t)m * ND + d) * N + i];
-        rs[d] = dm.rs[((size_t)m * ND + d) * N + i];
-      }
+      const int f = dm.cfd[(size_t)m * N + i];
       const double A = dm.af[(size_t)m * N + i];
-      double ui[NV], F[NV];
 #pragma unroll
       for (int v = 0; v < NV; v++) {
-        ui[v] = u[v];
-#pragma unroll
-        for (int d = 0; d < ND; d++) ui[v] += lg[v][d] * rs[d];
-      }
-      if (j >= 0) {
-        double rn[ND], uj[NV];
-#pragma unroll
-        for (int d = 0; d < ND; d++) rn[d] = dm.rn[((size_t)m * ND + d) * N + i];
-#pragma unroll
-        for (int v = 0; v < NV; v++) {
-          uj[v] = Us[(size_t)v * N + j];
-#pragma unroll
-          for (int d = 0; d < ND; d++) uj[v] += LG[((size_t)v * ND + d) * N + j] * rn[d];
-        }
-        if (own) roe_flux<ND, double>(gc, ui, uj, nn, F);
-        else roe_flux<ND, double>(gc, uj, ui, nn, F);
-      } else if (j == -1) {
-        slip_flux<ND, double>(gc, ui, nn, F);
-      } else {
-        freestream_flux<ND, double>(gc, ui, fi.p, nn, F);
-      }
-      bool bad = false;
-#pragma unroll
-      for (int v = 0; v < NV; v++) bad |= (F[v] != F[v]);
-      if (bad) atomicMin(err, (unsigned long long)i * K + m);
-#pragma unroll
-      for (int v = 0; v < NV; v++) {
-        if (own) R[v] += F[v] * A;
-        else R[v] -= F[v] * A;
+        const double x = FF[((size_t)f * NV + v) * B + b];
+        if (own) R[v] += x * A;
+        else R[v] -= x * A;
       }
     }
     if (Ud) {
-      const double c = alpha * (dtg ? dtg[0] : dt[i]) / dm.vol[i];
+      const double c = alpha * (dtg ? dtg[b] : dt[(size_t)i * B + b]) / dm.vol[i];
 #pragma unroll
-      for (int v = 0; v < NV; v++) Ud[(size_t)v * N + i] = U0[(size_t)v * N + i] - c * R[v];
+      for (int v = 0; v < NV; v++) {
+        const size_t q = ((size_t)v * N + i) * B + b;
+        Ud[q] = U0[q] - c * R[v];
+      }
     }
     if (Rout) {
 #pragma unroll
-      for (int v = 0; v < NV; v++) Rout[(size_t)v * N + i] = R[v];
+      for (int v = 0; v < NV; v++) Rout[((size_t)v * N + i) * B + b] = R[v];
     }
   }
   if (linf_bits) {
 #pragma unroll
     for (int v = 0; v < NV; v++) {
-      double a = block_max(::fabs(R[v]));
-      if (threadIdx.x == 0) atomicMax(&linf_bits[v], (unsigned long long)__double_as_longlong(a));
+      double a = block_max_cond(::fabs(R[v]), B);
+      if ((int)threadIdx.x < B)
+        atomicMax(&linf_bits[v * B + threadIdx.x], (unsigned long long)__double_as_longlong(a));
     }
   }
 }
@@ -302,9 +360,18 @@ __global__ void k_fill_u64(unsigned long long *p, int n, unsigned long long v) {
   if (i < n) p[i] = v;
 }
 
-__global__ void k_bits_to_double(const unsigned long long *bits, double *out, int n) {
-  int i = threadIdx.x;
-  if (i < n) out[i] = __longlong_as_double((long long)bits[i]);
+// Linf bit patterns [v][B] -> out[(it*B + b)*nv + v]; the row index comes from
+// the device counter *dit when given (graph replays), which is then advanced
+__global__ void k_linf_decode(const unsigned long long *bits, double *out, int nv, int B, int it, int cap,
+                              int *dit) {
+  const int q = threadIdx.x;
+  if (dit) it = *dit;
+  __syncthreads();
+  if (it < cap && q < nv * B) {
+    const int v = q / B, b = q - v * B;
+    out[((size_t)it * B + b) * nv + v] = __longlong_as_double((long long)bits[q]);
+  }
+  if (dit && q == 0) *dit = it + 1;
 }
 
 // per-face flux / Jacobian batches (host-facing known-answer interface)
@@ -394,10 +461,44 @@ static int build_device_mesh(bes_ctx *c) {
   c->iperm.assign(N, 0);
   for (int i = 0; i < N; i++) c->iperm[c->perm[i]] = i;
   std::vector<double> vol(N), eps2(N), sproj((size_t)nd * N), Sg((size_t)k * nd * N), w((size_t)k * N),
-      rl((size_t)k * nd * N), nf((size_t)k * nd * N), af((size_t)k * N), rs((size_t)k * nd * N),
-      rn((size_t)k * nd * N);
-  std::vector<int> nbr((size_t)k * N);
+      rl((size_t)k * nd * N), nf((size_t)k * nd * N), af((size_t)k * N);
+  std::vector<int> nbr((size_t)k * N), cfd((size_t)k * N);
   std::vector<uint32_t> info(N);
+  // faces in device order: by owner (Morton index), then the owner's local face
+  const int F = (int)g.nface;
+  std::vector<int64_t> dev2ref;
+  dev2ref.reserve(F);
+  std::vector<int> ref2dev(F, -1);
+  for (int i = 0; i < N; i++) {
+    const int64_t e = c->perm[i];
+    for (int m = 0; m < k; m++) {
+      const int64_t f = g.cface[e * k + m];
+      if (g.owner[f] == e && ref2dev[f] < 0) {
+        ref2dev[f] = (int)dev2ref.size();
+        dev2ref.push_back(f);
+      }
+    }
+  }
+  if ((int)dev2ref.size() != F) { set_error("face ownership is inconsistent"); return BES_ERR_MESH; }
+  std::vector<int> fo(F), fnb(F);
+  std::vector<double> fnr((size_t)nd * F), frs((size_t)nd * F), frn((size_t)nd * F, 0.0);
+  // reconstruction vector a cell uses for one of its faces (rsel: Q22 in compat mode)
+  auto recon = [&](int64_t cell, int64_t f, int d) -> double {
+    for (int mm = 0; mm < k; mm++)
+      if (g.cface[cell * k + mm] == f) return g.r[(cell * k + g.rsel[cell * k + mm]) * nd + d];
+    return 0.0;
+  };
+  for (int q = 0; q < F; q++) {
+    const int64_t f = dev2ref[q];
+    fo[q] = (int)c->iperm[g.owner[f]];
+    fnb[q] = g.nbr[f] >= 0 ? (int)c->iperm[g.nbr[f]] : (g.btype[f] == kSlipWall ? -1 : -2);
+    for (int d = 0; d < nd; d++) {
+      fnr[(size_t)d * F + q] = g.n[f * nd + d];
+      frs[(size_t)d * F + q] = recon(g.owner[f], f, d);
+      if (g.nbr[f] >= 0) frn[(size_t)d * F + q] = recon(g.nbr[f], f, d);
+    }
+  }
+  c->fref = dev2ref;
   for (int i = 0; i < N; i++) {
     const int64_t e = c->perm[i];
     vol[i] = g.vol[e];
@@ -422,31 +523,33 @@ static int build_device_mesh(bes_ctx *c) {
       nbr[(size_t)m * N + i] = code;
       w[(size_t)m * N + i] = g.w[s];
       af[(size_t)m * N + i] = g.area[f];
-      const int64_t sel = e * k + g.rsel[s];
-      // neighbour's own reconstruction vector for this face
-      int64_t nsel = -1;
-      if (jadj >= 0)
-        for (int mm = 0; mm < k; mm++)
-          if (g.cface[jadj * k + mm] == f) nsel = jadj * k + g.rsel[jadj * k + mm];
+      cfd[(size_t)m * N + i] = ref2dev[f];
       for (int d = 0; d < nd; d++) {
         const size_t q = ((size_t)m * nd + d) * N + i;
         Sg[q] = own ? g.S[f * nd + d] : -g.S[f * nd + d];
         rl[q] = g.r[s * nd + d];
         nf[q] = g.n[f * nd + d];
-        rs[q] = g.r[sel * nd + d];
-        rn[q] = nsel >= 0 ? g.r[nsel * nd + d] : 0.0;
       }
     }
     info[i] = bits;
   }
   DevMesh &dm = c->dm;
-  dm.nd = nd; dm.k = k; dm.N = N;
+  dm.nd = nd; dm.k = k; dm.N = N; dm.F = F;
   int s;
   if ((s = upload(&dm.vol, vol)) || (s = upload(&dm.eps2, eps2)) || (s = upload(&dm.sproj, sproj)) ||
       (s = upload(&dm.nbr, nbr)) || (s = upload(&dm.Sg, Sg)) || (s = upload(&dm.w, w)) ||
       (s = upload(&dm.rl, rl)) || (s = upload(&dm.nf, nf)) || (s = upload(&dm.af, af)) ||
-      (s = upload(&dm.rs, rs)) || (s = upload(&dm.rn, rn)) || (s = upload(&dm.info, info)))
+      (s = upload(&dm.cfd, cfd)) || (s = upload(&dm.info, info)) || (s = upload(&dm.fo, fo)) ||
+      (s = upload(&dm.fnb, fnb)) || (s = upload(&dm.fnr, fnr)) || (s = upload(&dm.frs, frs)) ||
+      (s = upload(&dm.frn, frn)))
     return s;
+  return BES_OK;
+}
+
+static int upload_finf(bes_ctx *c) {
+  BES_HIP(hipMemcpyAsync(c->dfinf, c->finf_b.data(), c->finf_b.size() * sizeof(double), hipMemcpyHostToDevice,
+                         c->stream));
+  BES_HIP(hipStreamSynchronize(c->stream));
   return BES_OK;
 }
 
@@ -471,8 +574,11 @@ static int create_from_input(const MeshInput &in, int geom_mode, const bes_gas *
   if (he != hipSuccess) { s = hip_fail(he, "hipStreamCreate"); delete c; return s; }
   const size_t N = c->N, nv = c->nv, nd = c->nd;
   if ((s = build_device_mesh(c)) || (s = dalloc(&c->U0, nv * N)) || (s = dalloc(&c->Ua, nv * N)) ||
-      (s = dalloc(&c->Ub, nv * N)) || (s = dalloc(&c->LG, nv * nd * N)) || (s = dalloc(&c->dt, N)) ||
-      (s = dalloc(&c->dtg, 1)) || (s = dalloc(&c->red, 64)) || (s = dalloc(&c->err, 1))) {
+      (s = dalloc(&c->Ub, nv * N)) || (s = dalloc(&c->LG, nv * (1 + nd) * N)) ||
+      (s = dalloc(&c->FF, nv * (size_t)c->dm.F)) || (s = dalloc(&c->dt, N)) ||
+      (s = dalloc(&c->dtg, bes_ctx::kMaxBatch)) || (s = dalloc(&c->red, bes_ctx::kRedWords)) ||
+      (s = dalloc(&c->err, 1)) || (s = dalloc(&c->dfinf, bes_ctx::kMaxBatch * 6)) ||
+      (s = dalloc(&c->dit, 1))) {
     bes_ctx_destroy(c);
     return s;
   }
@@ -481,6 +587,8 @@ static int create_from_input(const MeshInput &in, int geom_mode, const bes_gas *
   fi[nd + 1] = 1.0 / c->gc.gamma;
   fi[nd + 2] = 1.0 / (c->gc.gamma * c->gc.R);
   std::memcpy(c->finf, fi, sizeof(fi));
+  c->finf_b.assign(fi, fi + nv + 1);
+  if ((s = upload_finf(c))) { bes_ctx_destroy(c); return s; }
   *out = c;
   return BES_OK;
 }
@@ -496,9 +604,7 @@ static int check_err(bes_ctx *c) {
   BES_HIP(hipMemcpyAsync(&e, c->err, sizeof(e), hipMemcpyDeviceToHost, c->stream));
   BES_HIP(hipStreamSynchronize(c->stream));
   if (e != ~0ull) {
-    const int64_t cell = c->perm[e / c->k];
-    const int m = (int)(e % c->k);
-    c->last_err_face = (long)c->geo.cface[cell * c->k + m];
+    c->last_err_face = e < c->fref.size() ? (long)c->fref[e] : -1;   // device face -> reference id
     set_error("non-finite flux at face " + std::to_string(c->last_err_face));
     return BES_ERR_NAN;
   }
@@ -507,74 +613,141 @@ static int check_err(bes_ctx *c) {
 
 // launch helpers shared with the implicit path
 int launch_timestep(bes_ctx *c, const double *U, double cfl, int local) {
-  const int bs = 256, nb = grid_for(c->N, bs);
+  const int bs = 256, B = c->B, nb = grid_for(c->N * B, bs);
   unsigned long long *gm = nullptr;
   if (!local) {
-    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, c->stream, c->red, 1, 0x7ff0000000000000ull);
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, c->stream, c->red, B, 0x7ff0000000000000ull);
     gm = c->red;
   }
   {
-    ProfScope ps(c, kProfTimestep, (double)c->N * 8 * (c->nv + c->nd + 2));
-    if (c->nd == 2) hipLaunchKernelGGL(k_timestep<2>, dim3(nb), dim3(bs), 0, c->stream, U, c->dm, c->gc, cfl, c->dt, gm);
-    else hipLaunchKernelGGL(k_timestep<3>, dim3(nb), dim3(bs), 0, c->stream, U, c->dm, c->gc, cfl, c->dt, gm);
+    ProfScope ps(c, kProfTimestep, (double)c->N * (8.0 * (c->nv * B + c->nd + 1) + 8.0 * B));
+    if (c->nd == 2)
+      hipLaunchKernelGGL(k_timestep<2>, dim3(nb), dim3(bs), 0, c->stream, U, c->dm, c->gc, cfl, c->dt, gm, c->lgB);
+    else
+      hipLaunchKernelGGL(k_timestep<3>, dim3(nb), dim3(bs), 0, c->stream, U, c->dm, c->gc, cfl, c->dt, gm, c->lgB);
   }
-  if (!local) hipLaunchKernelGGL(k_decode_min, dim3(1), dim3(1), 0, c->stream, c->red, c->dtg);
+  if (!local) hipLaunchKernelGGL(k_decode_min, dim3(1), dim3(64), 0, c->stream, c->red, c->dtg, B);
   BES_HIP(hipGetLastError());
   return BES_OK;
+}
+
+// algorithmic HBM bytes of one gradient+limiter launch: state, geometry, pack
+double gradlim_bytes(const bes_ctx *c) {
+  const double N = c->N, B = c->B, nv = c->nv, nd = c->nd, k = c->k;
+  return N * (8 * nv * B + k * (4 + 16 * nd + 8) + 16 + 8 * nv * (1 + nd) * B);
 }
 
 int launch_gradlim(bes_ctx *c, const double *U, double *Gout, double *Pout) {
-  const int bs = 256, nb = grid_for(c->N, bs);
-  ProfScope ps(c, kProfGradlim,
-               (double)c->N * (8 * c->nv + c->k * (4 + 16 * c->nd + 8) + 16 + 8 * c->nv * c->nd));
-  if (c->nd == 2) hipLaunchKernelGGL(k_gradlim<2>, dim3(nb), dim3(bs), 0, c->stream, U, c->dm, c->LG, Gout, Pout);
-  else hipLaunchKernelGGL(k_gradlim<3>, dim3(nb), dim3(bs), 0, c->stream, U, c->dm, c->LG, Gout, Pout);
-  BES_HIP(hipGetLastError());
-  return BES_OK;
-}
-
-int launch_residual(bes_ctx *c, const double *Us, const double *U0, double *Ud, double alpha,
-                    int local, double *Rout, unsigned long long *linf_bits) {
-  const int bs = 256, nb = grid_for(c->N, bs);
-  Finf fi;
-  std::memcpy(fi.p, c->finf, sizeof(fi.p));
-  const double *dtg = local ? nullptr : c->dtg;
-  ProfScope ps(c, kProfResidual,
-               (double)c->N * (8 * c->nv + 8 * c->nv * c->nd + c->k * (12 + 24 * c->nd) + 20 +
-                               (U0 ? 16 * c->nv : 0) + (Rout ? 8 * c->nv : 0)));
+  const int bs = 256, nb = grid_for(c->N * c->B, bs);
+  ProfScope ps(c, kProfGradlim, gradlim_bytes(c));
   if (c->nd == 2)
-    hipLaunchKernelGGL(k_residual<2>, dim3(nb), dim3(bs), 0, c->stream, Us, c->LG, U0, Ud, c->dt, dtg,
-                       alpha, c->dm, c->gc, fi, Rout, linf_bits, c->err);
+    hipLaunchKernelGGL(k_gradlim<2>, dim3(nb), dim3(bs), 0, c->stream, U, c->dm, c->LG, Gout, Pout, c->lgB);
   else
-    hipLaunchKernelGGL(k_residual<3>, dim3(nb), dim3(bs), 0, c->stream, Us, c->LG, U0, Ud, c->dt, dtg,
-                       alpha, c->dm, c->gc, fi, Rout, linf_bits, c->err);
+    hipLaunchKernelGGL(k_gradlim<3>, dim3(nb), dim3(bs), 0, c->stream, U, c->dm, c->LG, Gout, Pout, c->lgB);
   BES_HIP(hipGetLastError());
   return BES_OK;
 }
 
-// device SoA (Morton order) <-> host cell-major (reference order)
+// face flux: face records, both cell packs (each pack read once overall), fluxes
+double faceflux_bytes(const bes_ctx *c) {
+  const double F = c->dm.F, N = c->N, B = c->B, nv = c->nv, nd = c->nd;
+  return F * (8 + 24 * nd + 8 * nv * B) + N * 8 * nv * (1 + nd) * B;
+}
+
+// face fluxes of the current packs (shared with the implicit assembly)
+int launch_face_flux(bes_ctx *c) {
+  const int bs = 256, nb = grid_for(c->dm.F * c->B, bs);
+  const int nd = c->nd;
+  ProfScope ps(c, kProfFaceFlux, faceflux_bytes(c));
+  if (nd == 2)
+    hipLaunchKernelGGL(k_face_flux<2>, dim3(nb), dim3(bs), 0, c->stream, c->LG, c->dm, c->gc, c->dfinf, c->FF,
+                       c->err, c->lgB);
+  else
+    hipLaunchKernelGGL(k_face_flux<3>, dim3(nb), dim3(bs), 0, c->stream, c->LG, c->dm, c->gc, c->dfinf, c->FF,
+                       c->err, c->lgB);
+  BES_HIP(hipGetLastError());
+  return BES_OK;
+}
+
+double residual_bytes(const bes_ctx *c, bool update, bool rout) {
+  const double N = c->N, B = c->B, nv = c->nv, k = c->k;
+  return N * (4 + k * 12 + (update ? 8 + 8 * B + 16 * nv * B : 0) + (rout ? 8 * nv * B : 0)) +
+         (double)c->dm.F * 8 * nv * B;
+}
+
+// residual of the packs of the last gradient pass; with U0/Ud the stage update
+int launch_residual(bes_ctx *c, const double *U0, double *Ud, double alpha, int local, double *Rout,
+                    unsigned long long *linf_bits) {
+  int s = launch_face_flux(c);
+  if (s) return s;
+  const int bs = 256, nb = grid_for(c->N * c->B, bs);
+  const double *dtg = local ? nullptr : c->dtg;
+  ProfScope ps(c, kProfResidual, residual_bytes(c, U0 != nullptr, Rout != nullptr));
+  if (c->nd == 2)
+    hipLaunchKernelGGL(k_cell_residual<2>, dim3(nb), dim3(bs), 0, c->stream, c->FF, U0, Ud, c->dt, dtg, alpha, c->dm,
+                       Rout, linf_bits, c->lgB);
+  else
+    hipLaunchKernelGGL(k_cell_residual<3>, dim3(nb), dim3(bs), 0, c->stream, c->FF, U0, Ud, c->dt, dtg, alpha, c->dm,
+                       Rout, linf_bits, c->lgB);
+  BES_HIP(hipGetLastError());
+  return BES_OK;
+}
+
+// device (Morton order, [comp][N][B]) <-> host (reference order, [B][ncell][comp])
 int state_to_host(bes_ctx *c, const double *dsoa, int ncomp, double *host) {
-  const size_t N = c->N;
-  std::vector<double> tmp(ncomp * N);
+  const size_t N = c->N, B = c->B;
+  std::vector<double> tmp(ncomp * N * B);
   BES_HIP(hipMemcpyAsync(tmp.data(), dsoa, tmp.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   BES_HIP(hipStreamSynchronize(c->stream));
-  for (size_t i = 0; i < N; i++) {
-    const int64_t e = c->perm[i];
-    for (int q = 0; q < ncomp; q++) host[e * ncomp + q] = tmp[(size_t)q * N + i];
-  }
+  for (size_t b = 0; b < B; b++)
+    for (size_t i = 0; i < N; i++) {
+      const int64_t e = c->perm[i];
+      for (int q = 0; q < ncomp; q++) host[(b * N + e) * ncomp + q] = tmp[((size_t)q * N + i) * B + b];
+    }
   return BES_OK;
 }
 
 int state_to_device(bes_ctx *c, const double *host, int ncomp, double *dsoa) {
-  const size_t N = c->N;
-  std::vector<double> tmp(ncomp * N);
-  for (size_t i = 0; i < N; i++) {
-    const int64_t e = c->perm[i];
-    for (int q = 0; q < ncomp; q++) tmp[(size_t)q * N + i] = host[e * ncomp + q];
-  }
+  const size_t N = c->N, B = c->B;
+  std::vector<double> tmp(ncomp * N * B);
+  for (size_t b = 0; b < B; b++)
+    for (size_t i = 0; i < N; i++) {
+      const int64_t e = c->perm[i];
+      for (int q = 0; q < ncomp; q++) tmp[((size_t)q * N + i) * B + b] = host[(b * N + e) * ncomp + q];
+    }
   BES_HIP(hipMemcpyAsync(dsoa, tmp.data(), tmp.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
   BES_HIP(hipStreamSynchronize(c->stream));
   return BES_OK;
+}
+
+static void drop_graph(bes_ctx *c) {
+  if (c->graph) hipGraphExecDestroy(c->graph);
+  c->graph = nullptr;
+}
+
+// (Re)allocate every per-condition array for a batch of B conditions
+static int resize_batch(bes_ctx *c, int B) {
+  if (c->stream) BES_HIP(hipStreamSynchronize(c->stream));
+  drop_graph(c);
+  double **arrs[] = {&c->U0, &c->Ua, &c->Ub, &c->LG, &c->FF, &c->dt, &c->G, &c->phi, &c->R, &c->linf};
+  for (double **p : arrs) {
+    if (*p) hipFree(*p);
+    *p = nullptr;
+  }
+  c->linf_cap = 0;
+  c->B = B;
+  c->lgB = 0;
+  while ((1 << c->lgB) < B) c->lgB++;
+  const size_t N = c->N, nv = c->nv, nd = c->nd, Bs = B;
+  int s;
+  if ((s = dalloc(&c->U0, nv * N * Bs)) || (s = dalloc(&c->Ua, nv * N * Bs)) || (s = dalloc(&c->Ub, nv * N * Bs)) ||
+      (s = dalloc(&c->LG, nv * (1 + nd) * N * Bs)) || (s = dalloc(&c->FF, nv * (size_t)c->dm.F * Bs)) ||
+      (s = dalloc(&c->dt, N * Bs)))
+    return s;
+  std::vector<double> fb((size_t)B * (nv + 1));
+  for (int b = 0; b < B; b++) std::copy(c->finf, c->finf + nv + 1, fb.begin() + (size_t)b * (nv + 1));
+  c->finf_b = fb;
+  return upload_finf(c);
 }
 
 }  // namespace bes
@@ -636,8 +809,10 @@ int bes_ctx_destroy(bes_ctx *c) {
   if (c->stream) hipStreamSynchronize(c->stream);
   bes_implicit_free(c);
   void *ptrs[] = {c->dm.vol, c->dm.eps2, c->dm.sproj, c->dm.nbr, c->dm.Sg, c->dm.w, c->dm.rl, c->dm.nf,
-                  c->dm.af, c->dm.rs, c->dm.rn, c->dm.info, c->U0, c->Ua, c->Ub, c->LG, c->G, c->phi,
-                  c->dt, c->dtg, c->R, c->red, c->err, c->linf};
+                  c->dm.af, c->dm.cfd, c->dm.info, c->dm.fo, c->dm.fnb, c->dm.fnr, c->dm.frs, c->dm.frn,
+                  c->FF, c->U0, c->Ua, c->Ub, c->LG, c->G, c->phi,
+                  c->dt, c->dtg, c->R, c->red, c->err, c->linf, c->dfinf, c->dit};
+  drop_graph(c);
   for (void *p : ptrs)
     if (p) hipFree(p);
   for (auto &p : c->ev_pending) { c->ev_pool.push_back(p.a); c->ev_pool.push_back(p.b); }
@@ -677,8 +852,38 @@ int bes_get_geometry(const bes_ctx *c, long *owner, long *nbr, int *btype, doubl
 
 int bes_set_freestream(bes_ctx *c, const double *prim) {
   if (!c || !prim) { set_error("null argument"); return BES_ERR_ARG; }
-  for (int q = 0; q < c->nv + 1; q++) c->finf[q] = prim[q];
-  return BES_OK;
+  const int nv = c->nv;
+  for (int q = 0; q < nv + 1; q++) c->finf[q] = prim[q];
+  for (int b = 0; b < c->B; b++) std::copy(prim, prim + nv + 1, c->finf_b.begin() + (size_t)b * (nv + 1));
+  return upload_finf(c);
+}
+
+int bes_set_batch(bes_ctx *c, int nbatch) {
+  if (!c) { set_error("null context"); return BES_ERR_ARG; }
+  if (nbatch < 1 || nbatch > bes_ctx::kMaxBatch || (nbatch & (nbatch - 1))) {
+    set_error("batch size must be a power of two in [1, 64]");
+    return BES_ERR_ARG;
+  }
+  if ((long)c->dm.F * nbatch > 0x7fffffffL || (long)c->N * nbatch > 0x7fffffffL) {
+    set_error("mesh too large for this batch size");
+    return BES_ERR_ARG;
+  }
+  if (nbatch > 1 && c->impl) {
+    set_error("the implicit path runs one condition; release it (bes_implicit_setup) first");
+    return BES_ERR_ARG;
+  }
+  if (nbatch == c->B) return BES_OK;
+  return resize_batch(c, nbatch);
+}
+
+int bes_get_batch(const bes_ctx *c) { return c ? c->B : 0; }
+
+int bes_set_freestream_batch(bes_ctx *c, const double *prim) {
+  if (!c || !prim) { set_error("null argument"); return BES_ERR_ARG; }
+  const int nv = c->nv;
+  std::copy(prim, prim + (size_t)c->B * (nv + 1), c->finf_b.begin());
+  for (int q = 0; q < nv + 1; q++) c->finf[q] = prim[q];
+  return upload_finf(c);
 }
 
 int bes_set_state(bes_ctx *c, const double *U) {
@@ -705,18 +910,18 @@ int bes_compute_time_step(bes_ctx *c, double cfl, int local) {
 int bes_get_time_step(bes_ctx *c, double *dt) {
   if (!c || !dt) { set_error("null argument"); return BES_ERR_ARG; }
   if (c->dt_local) return state_to_host(c, c->dt, 1, dt);
-  double g = 0;
-  BES_HIP(hipMemcpy(&g, c->dtg, sizeof(double), hipMemcpyDeviceToHost));
-  std::fill(dt, dt + c->N, g);
+  std::vector<double> g(c->B);
+  BES_HIP(hipMemcpy(g.data(), c->dtg, c->B * sizeof(double), hipMemcpyDeviceToHost));
+  for (int b = 0; b < c->B; b++) std::fill(dt + (size_t)b * c->N, dt + (size_t)(b + 1) * c->N, g[b]);
   return BES_OK;
 }
 
 int bes_compute_gradient_limiter(bes_ctx *c) {
   if (!c) { set_error("null context"); return BES_ERR_ARG; }
-  const size_t N = c->N;
+  const size_t N = c->N, B = c->B;
   if (!c->G) {
     int s;
-    if ((s = dalloc(&c->G, (size_t)c->nv * c->nd * N)) || (s = dalloc(&c->phi, (size_t)c->nv * N))) return s;
+    if ((s = dalloc(&c->G, (size_t)c->nv * c->nd * N * B)) || (s = dalloc(&c->phi, (size_t)c->nv * N * B))) return s;
   }
   int s = launch_gradlim(c, c->U0, c->G, c->phi);
   if (s) return s;
@@ -726,7 +931,7 @@ int bes_compute_gradient_limiter(bes_ctx *c) {
 
 int bes_get_gradient(bes_ctx *c, double *G) {
   if (!c || !G || !c->G) { set_error("gradient not computed"); return BES_ERR_ARG; }
-  // device [v][d][N] -> host [cell][v][d]
+  // device [v][d][N][B] -> host [B][cell][v][d]
   return state_to_host(c, c->G, c->nv * c->nd, G);
 }
 
@@ -735,56 +940,117 @@ int bes_get_limiter(bes_ctx *c, double *phi) {
   return state_to_host(c, c->phi, c->nv, phi);
 }
 
+static int fill_bits(bes_ctx *c, unsigned long long *p, int n, unsigned long long v) {
+  hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(n, 64)), dim3(64), 0, c->stream, p, n, v);
+  BES_HIP(hipGetLastError());
+  return BES_OK;
+}
+
 int bes_compute_residual(bes_ctx *c, double *R_out, double *maxabs_out) {
   if (!c) { set_error("null context"); return BES_ERR_ARG; }
-  if (!c->R) { int s = dalloc(&c->R, (size_t)c->nv * c->N); if (s) return s; }
+  if (!c->R) { int s = dalloc(&c->R, (size_t)c->nv * c->N * c->B); if (s) return s; }
+  const int nv = c->nv, B = c->B;
+  unsigned long long *bits = c->red + 64;
   int s;
-  if ((s = reset_err(c))) return s;
-  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, c->stream, c->red, 8, 0ull);
-  if ((s = launch_residual(c, c->U0, nullptr, nullptr, 0.0, 1, c->R, c->red))) return s;
+  if ((s = reset_err(c)) || (s = fill_bits(c, bits, nv * B, 0ull))) return s;
+  if ((s = launch_residual(c, nullptr, nullptr, 0.0, 1, c->R, bits))) return s;
   if (maxabs_out) {
-    unsigned long long b[8];
-    BES_HIP(hipMemcpyAsync(b, c->red, sizeof(b), hipMemcpyDeviceToHost, c->stream));
+    std::vector<unsigned long long> b(nv * B);
+    BES_HIP(hipMemcpyAsync(b.data(), bits, b.size() * 8, hipMemcpyDeviceToHost, c->stream));
     BES_HIP(hipStreamSynchronize(c->stream));
-    for (int v = 0; v < c->nv; v++) { double x; std::memcpy(&x, &b[v], 8); maxabs_out[v] = x; }
+    for (int q = 0; q < B; q++)
+      for (int v = 0; v < nv; v++) std::memcpy(&maxabs_out[q * nv + v], &b[v * B + q], 8);
   }
   if (R_out && (s = state_to_host(c, c->R, c->nv, R_out))) return s;
   return check_err(c);
 }
 
+// one pseudo-time step (SolverExplicit.cpp:259-393) on the context stream;
+// it_host < 0: the Linf row index comes from the device counter
+static int explicit_iteration(bes_ctx *c, int nstage, const double *alpha, double cfl, int local, bool linf,
+                              int it_host) {
+  const int nv = c->nv, B = c->B;
+  unsigned long long *bits = c->red + 64;
+  int s;
+  if ((s = launch_timestep(c, c->U0, cfl, local))) return s;
+  for (int st = 0; st < nstage; st++) {
+    const bool last = st == nstage - 1;
+    const double *src = st == 0 ? c->U0 : ((st - 1) % 2 == 0 ? c->Ua : c->Ub);
+    double *dst = (last && nstage > 1) ? c->U0 : (st % 2 == 0 ? c->Ua : c->Ub);
+    if ((s = launch_gradlim(c, src, nullptr, nullptr))) return s;
+    if (last && linf && (s = fill_bits(c, bits, nv * B, 0ull))) return s;
+    if ((s = launch_residual(c, c->U0, dst, alpha[st], local, nullptr, (last && linf) ? bits : nullptr))) return s;
+    if (last && linf) {
+      hipLaunchKernelGGL(k_linf_decode, dim3(1), dim3(nv * B), 0, c->stream, bits, c->linf, nv, B,
+                         it_host < 0 ? 0 : it_host, c->linf_cap, it_host < 0 ? c->dit : nullptr);
+      BES_HIP(hipGetLastError());
+    }
+    if (last && nstage == 1) std::swap(c->U0, c->Ua);
+  }
+  return BES_OK;
+}
+
+// capture one pseudo-time step as a graph (stage count > 1: the slots are fixed)
+static int capture_iteration(bes_ctx *c, int nstage, const double *alpha, double cfl, int local, bool linf) {
+  const int key[4] = {nstage, local, linf ? 1 : 0, c->B};
+  if (c->graph && std::equal(key, key + 4, c->graph_key) && c->graph_cfl == cfl &&
+      std::equal(alpha, alpha + nstage, c->graph_alpha.begin(), c->graph_alpha.end()))
+    return BES_OK;
+  drop_graph(c);
+  const bool prof = c->prof;
+  c->prof = false;   // no event records inside the capture
+  hipGraph_t g = nullptr;
+  BES_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  int s = explicit_iteration(c, nstage, alpha, cfl, local, linf, -1);
+  hipError_t e = hipStreamEndCapture(c->stream, &g);
+  c->prof = prof;
+  if (s) { if (g) hipGraphDestroy(g); return s; }
+  if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+  e = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
+  hipGraphDestroy(g);
+  if (e != hipSuccess) { c->graph = nullptr; return hip_fail(e, "hipGraphInstantiate"); }
+  std::copy(key, key + 4, c->graph_key);
+  c->graph_cfl = cfl;
+  c->graph_alpha.assign(alpha, alpha + nstage);
+  return BES_OK;
+}
+
+int bes_explicit_graph(bes_ctx *c, int on) {
+  if (!c) { set_error("null context"); return BES_ERR_ARG; }
+  c->use_graph = on != 0;
+  if (!on) drop_graph(c);
+  return BES_OK;
+}
+
 int bes_explicit_run(bes_ctx *c, int niter, int nstage, const double *alpha, double cfl, int local,
                      double *linf_out) {
   if (!c || niter < 0 || nstage < 1 || nstage > 16 || !alpha) { set_error("invalid argument"); return BES_ERR_ARG; }
-  const int nv = c->nv;
+  const int nv = c->nv, B = c->B;
   if (linf_out && c->linf_cap < niter) {
     if (c->linf) hipFree(c->linf);
     c->linf = nullptr;
     c->linf_cap = 0;
-    int s = dalloc(&c->linf, (size_t)niter * nv);
+    drop_graph(c);   // the capture holds the old Linf buffer
+    int s = dalloc(&c->linf, (size_t)niter * nv * B);
     if (s) return s;
     c->linf_cap = niter;
   }
   int s;
   if ((s = reset_err(c))) return s;
-  unsigned long long *bits = c->red + 8;
-  for (int it = 0; it < niter; it++) {
-    if ((s = launch_timestep(c, c->U0, cfl, local))) return s;
-    for (int st = 0; st < nstage; st++) {
-      const bool last = st == nstage - 1;
-      const double *src = st == 0 ? c->U0 : ((st - 1) % 2 == 0 ? c->Ua : c->Ub);
-      double *dst = (last && nstage > 1) ? c->U0 : (st % 2 == 0 ? c->Ua : c->Ub);
-      if ((s = launch_gradlim(c, src, nullptr, nullptr))) return s;
-      if (last && linf_out) hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, c->stream, bits, nv, 0ull);
-      if ((s = launch_residual(c, src, c->U0, dst, alpha[st], local, nullptr, (last && linf_out) ? bits : nullptr)))
-        return s;
-      if (last && linf_out)
-        hipLaunchKernelGGL(k_bits_to_double, dim3(1), dim3(64), 0, c->stream, bits, c->linf + (size_t)it * nv, nv);
-      if (last && nstage == 1) std::swap(c->U0, c->Ua);
-    }
+  c->dt_local = local != 0;
+  const bool graph = c->use_graph && nstage > 1 && !c->prof;
+  if (graph) {
+    if ((s = capture_iteration(c, nstage, alpha, cfl, local, linf_out != nullptr))) return s;
+    BES_HIP(hipMemsetAsync(c->dit, 0, sizeof(int), c->stream));
+    for (int it = 0; it < niter; it++) BES_HIP(hipGraphLaunch(c->graph, c->stream));
+  } else {
+    for (int it = 0; it < niter; it++)
+      if ((s = explicit_iteration(c, nstage, alpha, cfl, local, linf_out != nullptr, it))) return s;
   }
   if ((s = prof_collect(c))) return s;
   if (linf_out && niter > 0) {
-    BES_HIP(hipMemcpyAsync(linf_out, c->linf, (size_t)niter * nv * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    BES_HIP(hipMemcpyAsync(linf_out, c->linf, (size_t)niter * nv * B * sizeof(double), hipMemcpyDeviceToHost,
+                           c->stream));
   }
   BES_HIP(hipStreamSynchronize(c->stream));
   return check_err(c);

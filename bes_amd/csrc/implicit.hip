@@ -39,6 +39,7 @@
 namespace bes {
 
 int launch_timestep(bes_ctx *c, const double *U, double cfl, int local);
+int launch_face_flux(bes_ctx *c);
 int launch_gradlim(bes_ctx *c, const double *U, double *Gout, double *Pout);
 int state_to_host(bes_ctx *c, const double *dsoa, int ncomp, double *host);
 int state_to_device(bes_ctx *c, const double *host, int ncomp, double *dsoa);
@@ -130,11 +131,9 @@ __global__ __launch_bounds__(256) void k_face_jacobian(const double *__restrict_
 // ---------------------------------------------------------------------------
 template <int ND, typename TM>
 __global__ __launch_bounds__(256) void k_assemble(
-    const double *__restrict__ U, const double *__restrict__ LG, const double *__restrict__ dt,
-    const double *__restrict__ dtg, DevMesh dm, GasConst gc, FinfI fi, FaceDev fd,
-    const double *__restrict__ JF, int repeat, double *__restrict__ b, TM *__restrict__ A,
-    unsigned long long *__restrict__ linf_b, unsigned long long *__restrict__ linf_r,
-    unsigned long long *__restrict__ err) {
+    const double *__restrict__ FF, const double *__restrict__ dt, const double *__restrict__ dtg, DevMesh dm,
+    FaceDev fd, const double *__restrict__ JF, int repeat, double *__restrict__ b, TM *__restrict__ A,
+    unsigned long long *__restrict__ linf_b, unsigned long long *__restrict__ linf_r) {
   constexpr int NV = ND + 2, K = ND + 1, NB = NV * NV;
   const int N = dm.N;
   const int i = cell_index();
@@ -144,58 +143,20 @@ __global__ __launch_bounds__(256) void k_assemble(
   if (i < N) {
     const double mdt = (dtg ? dtg[0] : dt[i]) / dm.vol[i];
     const uint32_t info = dm.info[i];
-    double u[NV], lg[NV][ND];
+    // right-hand side (SolverImplicit.cpp:441-473) from the second-order face
+    // fluxes of k_face_flux, faces in reference order
 #pragma unroll
-    for (int v = 0; v < NV; v++) {
-      u[v] = U[(size_t)v * N + i];
-#pragma unroll
-      for (int d = 0; d < ND; d++) lg[v][d] = LG[((size_t)v * ND + d) * N + i];
-    }
-    // right-hand side (SolverImplicit.cpp:441-473), faces in reference order
     for (int q = 0; q < K; q++) {
       const int m = (info >> (2 * q)) & 3;
       const bool own = (info >> (8 + m)) & 1;
-      const int j = dm.nbr[(size_t)m * N + i];
-      double nn[ND], rs[ND];
-#pragma unroll
-      for (int d = 0; d < ND; d++) {
-        nn[d] = dm.nf[((size_t)m * ND + d) * N + i];
-        rs[d] = dm.rs[((size_t)m * ND + d) * N + i];
-      }
+      const int fdv = dm.cfd[(size_t)m * N + i];
       const double Af = dm.af[(size_t)m * N + i];
-      double ui[NV], F[NV];
-#pragma unroll
-      for (int v = 0; v < NV; v++) {
-        ui[v] = u[v];
-#pragma unroll
-        for (int d = 0; d < ND; d++) ui[v] += lg[v][d] * rs[d];
-      }
-      if (j >= 0) {
-        double rn[ND], uj[NV];
-#pragma unroll
-        for (int d = 0; d < ND; d++) rn[d] = dm.rn[((size_t)m * ND + d) * N + i];
-#pragma unroll
-        for (int v = 0; v < NV; v++) {
-          uj[v] = U[(size_t)v * N + j];
-#pragma unroll
-          for (int d = 0; d < ND; d++) uj[v] += LG[((size_t)v * ND + d) * N + j] * rn[d];
-        }
-        if (own) roe_flux<ND, double>(gc, ui, uj, nn, F);
-        else roe_flux<ND, double>(gc, uj, ui, nn, F);
-      } else if (j == -1) {
-        slip_flux<ND, double>(gc, ui, nn, F);
-      } else {
-        freestream_flux<ND, double>(gc, ui, fi.p, nn, F);
-      }
-      bool bad = false;
-#pragma unroll
-      for (int v = 0; v < NV; v++) bad |= (F[v] != F[v]);
-      if (bad) atomicMin(err, (unsigned long long)i * K + m);
       const double mul = mdt * Af;
 #pragma unroll
       for (int v = 0; v < NV; v++) {
-        if (own) { bi[v] -= mul * F[v]; ri[v] += F[v] * Af; }
-        else { bi[v] += mdt * Af * F[v]; ri[v] -= F[v] * Af; }
+        const double F = FF[(size_t)fdv * NV + v];
+        if (own) { bi[v] -= mul * F; ri[v] += F * Af; }
+        else { bi[v] += mdt * Af * F; ri[v] -= F * Af; }
       }
     }
 #pragma unroll
@@ -766,21 +727,20 @@ static int launch_assemble(bes_ctx *c, Impl *im) {
     BES_HIP(hipGetLastError());
   }
   hipLaunchKernelGGL(k_fill_u64_i, dim3(1), dim3(64), 0, c->stream, im->bits, 16, 0ull);
-  // per cell: state, limited gradient, dt, volume, face records, both Jacobian
-  // blocks of each face; writes b and k+1 matrix blocks
+  int s = launch_face_flux(c);
+  if (s) return s;
+  // per cell: dt, volume, face records, the cell's face fluxes and both
+  // Jacobian blocks of each face; writes b and k+1 matrix blocks
   ProfScope ps(c, kProfAssemble,
-               (double)c->N * (8.0 * c->nv + 8.0 * c->nv * c->nd + 20.0 + c->k * (16.0 + 24.0 * c->nd) +
-                               8.0 * c->nv) +
+               (double)c->N * (20.0 + c->k * 12.0 + 8.0 * c->nv) + (double)c->dm.F * 8.0 * c->nv +
                    (double)(im->fd.F + im->ninternal) * 2.0 * nbl * 8.0 +
                    (double)c->N * (c->k + 1) * nbl * sizeof(TM));
   if (c->nd == 2)
-    hipLaunchKernelGGL((k_assemble<2, TM>), dim3(nb), dim3(bs), 0, c->stream, c->U0, c->LG, c->dt, dtg, c->dm,
-                       c->gc, fi, im->fd, im->JF, im->repeat_nv, im->b, (TM *)im->A, im->bits, im->bits + 8,
-                       c->err);
+    hipLaunchKernelGGL((k_assemble<2, TM>), dim3(nb), dim3(bs), 0, c->stream, c->FF, c->dt, dtg, c->dm, im->fd,
+                       im->JF, im->repeat_nv, im->b, (TM *)im->A, im->bits, im->bits + 8);
   else
-    hipLaunchKernelGGL((k_assemble<3, TM>), dim3(nb), dim3(bs), 0, c->stream, c->U0, c->LG, c->dt, dtg, c->dm,
-                       c->gc, fi, im->fd, im->JF, im->repeat_nv, im->b, (TM *)im->A, im->bits, im->bits + 8,
-                       c->err);
+    hipLaunchKernelGGL((k_assemble<3, TM>), dim3(nb), dim3(bs), 0, c->stream, c->FF, c->dt, dtg, c->dm, im->fd,
+                       im->JF, im->repeat_nv, im->b, (TM *)im->A, im->bits, im->bits + 8);
   BES_HIP(hipGetLastError());
   return BES_OK;
 }
@@ -1041,6 +1001,10 @@ int bes_implicit_setup(bes_ctx *c, int precision, int precond, int restart, int 
     set_error("invalid implicit setup argument (precision 64/32, precond 0-2, restart 1-30)");
     return BES_ERR_ARG;
   }
+  if (c->B != 1) {
+    set_error("the implicit path runs one condition (batch size 1)");
+    return BES_ERR_ARG;
+  }
   bes_implicit_free(c);
   Impl *im = new Impl();
   im->prec = precision;
@@ -1078,8 +1042,7 @@ int bes_implicit_assemble(bes_ctx *c, double cfl, int local, int repeat_nv, doub
   unsigned long long e = 0;
   BES_HIP(hipMemcpy(&e, c->err, sizeof(e), hipMemcpyDeviceToHost));
   if (e != ~0ull) {
-    const int64_t cell = c->perm[e / c->k];
-    c->last_err_face = (long)c->geo.cface[cell * c->k + (int)(e % c->k)];
+    c->last_err_face = e < c->fref.size() ? (long)c->fref[e] : -1;   // device face -> reference id
     set_error("non-finite flux at face " + std::to_string(c->last_err_face));
     return BES_ERR_NAN;
   }

@@ -61,4 +61,34 @@ __device__ __forceinline__ double block_min(double v) {
   return r;
 }
 
+// Per-condition block reductions for batched launches: thread t of a block
+// works on condition t % B (B a power of two <= 64, blockDim.x a multiple of
+// 64).  Lanes of a wave with the same condition are combined by xor-shuffles
+// with offsets >= B, then the waves through LDS; thread t < B holds the
+// result for condition t.  Every thread of the block must call them.
+__device__ __forceinline__ double block_max_cond(double v, int B) {
+  __shared__ double s[16][64];
+  for (int o = 32; o >= B; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  __syncthreads();
+  if (l < B) s[w][l] = v;
+  __syncthreads();
+  double r = -__builtin_inf();
+  if ((int)threadIdx.x < B)
+    for (int q = 0; q < (int)(blockDim.x >> 6); q++) r = fmax(r, s[q][threadIdx.x]);
+  return r;
+}
+__device__ __forceinline__ double block_min_cond(double v, int B) {
+  __shared__ double s[16][64];
+  for (int o = 32; o >= B; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  __syncthreads();
+  if (l < B) s[w][l] = v;
+  __syncthreads();
+  double r = __builtin_inf();
+  if ((int)threadIdx.x < B)
+    for (int q = 0; q < (int)(blockDim.x >> 6); q++) r = fmin(r, s[q][threadIdx.x]);
+  return r;
+}
+
 }  // namespace bes

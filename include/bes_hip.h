@@ -78,35 +78,57 @@ int bes_get_geometry(const bes_ctx *ctx, long *owner, long *nbr, int *btype, dou
                      double *n, double *area, double *vol, double *centroid,
                      long *cell_faces, long *cell_adj, double *w, double *r);
 
-/* freestream primitive [nv+1] used by far-field faces (Config.cpp:479-488,688-706) */
+/* freestream primitive [nv+1] used by far-field faces (Config.cpp:479-488,688-706);
+ * sets every condition of the batch */
 int bes_set_freestream(bes_ctx *ctx, const double *primitive);
-/* SolutionSnapshot::setConservedVariable / getConservedVariable for all cells */
+
+/* ---- batched conditions (no reference counterpart: the reference runs one
+ *      freestream condition per process, Config.cpp:479-488).  nbatch, a power
+ *      of two in [1, 64], freestream conditions share the mesh and advance
+ *      together; every per-cell host array of the state, time-step, gradient,
+ *      limiter and residual calls below then holds nbatch blocks, condition-major
+ *      ([nbatch][ncell][...]), and per-variable outputs hold [nbatch][nv].
+ *      Changing the batch size discards the device state.  The implicit path
+ *      requires nbatch = 1. ---- */
+int bes_set_batch(bes_ctx *ctx, int nbatch);
+int bes_get_batch(const bes_ctx *ctx);
+/* freestream primitive per condition, [nbatch][nv+1] */
+int bes_set_freestream_batch(bes_ctx *ctx, const double *primitive);
+
+/* SolutionSnapshot::setConservedVariable / getConservedVariable for all cells
+ * (SolutionSnapshot.cpp:60-143); U [nbatch][ncell][nv] */
 int bes_set_state(bes_ctx *ctx, const double *U);
 int bes_get_state(bes_ctx *ctx, double *U);
 
 /* ---- TimeStepCalculator::computeAllTimeStep (TimeStepCalculator.hpp:43-46;
  *      TimeStepCalculatorLocal.cpp:67-93, TimeStepCalculatorGlobal.cpp:64-98) ---- */
 int bes_compute_time_step(bes_ctx *ctx, double cfl, int local);
-int bes_get_time_step(bes_ctx *ctx, double *dt /*[ncell]*/);
+int bes_get_time_step(bes_ctx *ctx, double *dt /*[nbatch][ncell]*/);
 
 /* ---- Gradient::computeGradient + Limiter::computeLimiterTerm (Gradient.hpp:44-49,
  *      Limiter.hpp:45-51; GradientGreenGauss.cpp:21-135,
  *      LimiterVenkatakrishnan.cpp:21-128): one fused pass over the current state ---- */
 int bes_compute_gradient_limiter(bes_ctx *ctx);
-int bes_get_gradient(bes_ctx *ctx, double *G /*[ncell][nv][ndim]*/);
-int bes_get_limiter(bes_ctx *ctx, double *phi /*[ncell][nv]*/);
+int bes_get_gradient(bes_ctx *ctx, double *G /*[nbatch][ncell][nv][ndim]*/);
+int bes_get_limiter(bes_ctx *ctx, double *phi /*[nbatch][ncell][nv]*/);
 
 /* ---- second-order residual sum_f +-F*A per cell (the face loop of
  *      SolverExplicit.cpp:285-344 with Solver.cpp:69-121); uses the gradient and
- *      limiter of the last bes_compute_gradient_limiter call.  R_out [ncell][nv]
- *      may be NULL; maxabs_out [nv] (Linf per variable) may be NULL ---- */
+ *      limiter of the last bes_compute_gradient_limiter call.  R_out
+ *      [nbatch][ncell][nv] may be NULL; maxabs_out [nbatch][nv] (Linf per
+ *      variable) may be NULL ---- */
 int bes_compute_residual(bes_ctx *ctx, double *R_out, double *maxabs_out);
 
 /* ---- SolverExplicit::timeStep_ (SolverExplicit.cpp:259-393): niter full
- *      multi-stage pseudo-time steps on the device state.  linf_out [niter][nv]
- *      (last-stage Linf of the residual, the reference monitor) may be NULL. ---- */
+ *      multi-stage pseudo-time steps on the device state.  linf_out
+ *      [niter][nbatch][nv] (last-stage Linf of the residual, the reference
+ *      monitor) may be NULL. ---- */
 int bes_explicit_run(bes_ctx *ctx, int niter, int nstage, const double *alpha, double cfl,
                      int local, double *linf_out);
+/* on != 0: bes_explicit_run captures one pseudo-time step (time step, every
+ * stage, the monitor) as a HIP graph and replays it niter times, one graph
+ * launch per step (stage counts > 1, profiling off; no reference counterpart) */
+int bes_explicit_graph(bes_ctx *ctx, int on);
 
 /* ---- Flux_::computeFlux / FluxJacobian::computeJacobian over a batch of faces
  *      (Flux.hpp:43-46, FluxJacobian.hpp:36-44; FluxRoe.tpp:306-333,
