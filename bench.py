@@ -44,7 +44,7 @@ KERNEL_NAMES = {"timestep": "k_timestep", "gradlim": "k_gradlim", "residual": "k
                 "faceflux": "k_face_flux",
                 "jacobian": "k_face_jacobian", "assemble": "k_assemble", "spmv": "k_spmv",
                 "precond": "k_ilu_lower+k_ilu_upper", "krylov": "k_mdot+k_orth_update",
-                "factor": "k_ilu_factor_color", "update": "k_update"}
+                "factor": "k_ilu_factor_color", "update": "k_update", "learned": "learned preconditioner kernels"}
 
 
 def log(*a):

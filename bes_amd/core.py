@@ -77,9 +77,22 @@ OPTIONAL_SIGS = {
     "bes_set_profiling": [C.c_void_p, C.c_int],
     "bes_get_profile": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int],
     "bes_synchronize": [C.c_void_p],
+    "bes_implicit_set_precond": [C.c_void_p, C.c_int],
+    "bes_set_batch": [C.c_void_p, C.c_int],
+    "bes_get_batch": [C.c_void_p],
+    "bes_set_freestream_batch": [C.c_void_p, C.c_void_p],
+    "bes_explicit_graph": [C.c_void_p, C.c_int],
+    "bes_lp_param_count": [],
+    "bes_lp_init": [C.c_void_p, C.c_ulonglong],
+    "bes_lp_set_params": [C.c_void_p, C.c_void_p, C.c_int],
+    "bes_lp_get_params": [C.c_void_p, C.c_void_p, C.c_int],
+    "bes_lp_operator": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],
+    "bes_lp_loss_grad": [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p],
+    "bes_lp_train": [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_int, C.c_ulonglong, C.c_void_p],
+    "bes_lp_fallbacks": [C.c_void_p],
 }
 PROF_KINDS = ["timestep", "gradlim", "residual", "jacobian", "assemble", "spmv", "precond",
-              "krylov", "factor", "update", "faceflux"]
+              "krylov", "factor", "update", "faceflux", "learned"]
 
 
 def lib():
@@ -254,10 +267,13 @@ class Context:
         _check(lib().bes_synchronize(self.h))
 
     # ---- implicit path ----
-    PRECOND = {"none": 0, "jacobi": 1, "ilu0": 2}
+    PRECOND = {"none": 0, "jacobi": 1, "ilu0": 2, "learned": 3}
 
     def implicit_setup(self, precision=64, precond="ilu0", restart=30, maxiter=1000, rtol=1e-7):
         _check(lib().bes_implicit_setup(self.h, precision, self.PRECOND[precond], restart, maxiter, rtol))
+
+    def implicit_set_precond(self, precond):
+        _check(lib().bes_implicit_set_precond(self.h, self.PRECOND[precond]))
 
     def implicit_assemble(self, cfl, local=True, repeat_nv=True):
         lb = np.zeros(self.nv)
@@ -300,6 +316,44 @@ class Context:
         _check(lib().bes_implicit_run(self.h, nsteps, cfl_min, cfl_max, _p(ex), int(local), int(repeat_nv),
                                       stop_drop, _p(hist), C.byref(done)), self.h)
         return hist[:done.value]
+
+    # ---- learned preconditioner ----
+    def lp_init(self, seed=20261202):
+        _check(lib().bes_lp_init(self.h, seed))
+
+    def lp_params(self):
+        p = np.zeros(lib().bes_lp_param_count(), np.float32)
+        _check(lib().bes_lp_get_params(self.h, _p(p), p.size))
+        return p
+
+    def lp_set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        _check(lib().bes_lp_set_params(self.h, _p(p), p.size))
+
+    def lp_operator(self):
+        """Blocks of the learned operator M for the assembled system (reference order)."""
+        nv, k = self.nv, self.k
+        Md = np.zeros((self.ncell, nv, nv))
+        Me = np.zeros((self.ncell, k, nv, nv))
+        G = np.zeros((self.ncell, k, nv))
+        _check(lib().bes_lp_operator(self.h, _p(Md), _p(Me), _p(G)), self.h)
+        return Md, Me, G
+
+    def lp_loss_grad(self, probes, nrich=3):
+        V = np.ascontiguousarray(probes, np.float64)
+        assert V.ndim == 3 and V.shape[1:] == (self.ncell, self.nv)
+        loss = C.c_double()
+        g = np.zeros(lib().bes_lp_param_count(), np.float32)
+        _check(lib().bes_lp_loss_grad(self.h, _p(V), V.shape[0], nrich, C.byref(loss), _p(g)), self.h)
+        return loss.value, g
+
+    def lp_train(self, steps, lr=3e-3, nprobe=4, nrich=3, seed=1):
+        hist = np.zeros(steps)
+        _check(lib().bes_lp_train(self.h, steps, lr, nprobe, nrich, seed, _p(hist)), self.h)
+        return hist
+
+    def lp_fallbacks(self):
+        return lib().bes_lp_fallbacks(self.h)
 
     def explicit_graph(self, on=True):
         """Replay each pseudo-time step as one captured HIP graph."""

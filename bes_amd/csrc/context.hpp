@@ -113,7 +113,7 @@ inline int grid_for(int n, int bs) { return (n + bs - 1) / bs; }
 // kernel kinds for per-kernel timing
 enum ProfKind { kProfTimestep = 0, kProfGradlim = 1, kProfResidual = 2, kProfJacobian = 3,
                 kProfAssemble = 4, kProfSpmv = 5, kProfPrecond = 6, kProfKrylov = 7, kProfFactor = 8,
-                kProfUpdate = 9, kProfFaceFlux = 10 };
+                kProfUpdate = 9, kProfFaceFlux = 10, kProfLearned = 11 };
 // bracket a launch with events when profiling is on
 // bytes: the launch's algorithmic HBM bytes (every array it must touch, once)
 struct ProfScope {

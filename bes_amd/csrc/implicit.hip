@@ -34,6 +34,7 @@
 #include "../../include/bes_hip.h"
 #include "context.hpp"
 #include "flux.hpp"
+#include "implicit.hpp"
 #include "kernels_common.hpp"
 
 namespace bes {
@@ -43,49 +44,10 @@ int launch_face_flux(bes_ctx *c);
 int launch_gradlim(bes_ctx *c, const double *U, double *Gout, double *Pout);
 int state_to_host(bes_ctx *c, const double *dsoa, int ncomp, double *host);
 int state_to_device(bes_ctx *c, const double *host, int ncomp, double *dsoa);
-
-constexpr int kRedBlocks = 1024;   // partial-sum blocks of the reductions
-
-struct FaceDev {
-  int F = 0;
-  int *fo = nullptr;       // [F] owner (device order)
-  int *fn = nullptr;       // [F] neighbour, -1 slip wall, -2 far field
-  double *fnrm = nullptr;  // [nd][F] unit normal
-  int *cf = nullptr;       // [k][N] face id of (cell, local face)
-  int *mrev = nullptr;     // [k][N] local index of this face in the neighbour
-};
-
-template <typename TM>
-struct Mat {
-  TM *val = nullptr;       // [(k+1)*nv*nv][N]
-};
-
-struct Impl {
-  int prec = 64;           // 64 or 32
-  int precond = 2;         // 0 none, 1 block Jacobi, 2 multicolour ILU(0)
-  int restart = 30, maxiter = 1000;
-  double rtol = 1e-7;
-  int repeat_nv = 1;
-  FaceDev fd;
-  double *JF = nullptr;    // [F][2][nv*nv]
-  double *b = nullptr;     // [N][nv] FP64 right-hand side
-  double *x = nullptr;     // [N][nv] FP64 solution
-  void *A = nullptr;       // matrix values (TM)
-  void *LU = nullptr;      // factor / inverse diagonal blocks (TM), [(k+1)*nv*nv][N]
-  void *V = nullptr;       // Krylov basis (TV) [(m+1)][N*nv]
-  void *Z = nullptr;       // work vectors (TV) [3][N*nv]
-  double *part = nullptr;  // reduction partials [32][kRedBlocks]
-  double *dots = nullptr;  // [64]
-  unsigned long long *bits = nullptr;   // [16] Linf monitors
-  int ncolor = 0;
-  std::vector<int> color_start;   // host: rows of colour c are crow[color_start[c] .. color_start[c+1])
-  int *crow = nullptr;            // [N] rows grouped by colour
-  int *color = nullptr;           // [N]
-  long ninternal = 0;              // internal faces (off-diagonal block pairs)
-  int last_iters = 0;
-  double last_relres = 0;
-  bool assembled = false, factored = false;
-};
+namespace lp {
+int lp_build(bes_ctx *c, Impl *im);
+int lp_release(Impl *im);
+}  // namespace lp
 
 // ---------------------------------------------------------------------------
 // Jacobians
@@ -211,59 +173,6 @@ __global__ __launch_bounds__(256) void k_assemble(
   }
 }
 
-// ---------------------------------------------------------------------------
-// small dense block helpers (NV x NV, row-major, registers)
-// ---------------------------------------------------------------------------
-template <int NV>
-__device__ __forceinline__ void block_inverse(double *A, double *Inv) {
-  // Gauss-Jordan with partial pivoting
-#pragma unroll
-  for (int r = 0; r < NV; r++)
-#pragma unroll
-    for (int c = 0; c < NV; c++) Inv[r * NV + c] = (r == c) ? 1.0 : 0.0;
-#pragma unroll
-  for (int p = 0; p < NV; p++) {
-    int piv = p;
-    double best = ::fabs(A[p * NV + p]);
-#pragma unroll
-    for (int r = p + 1; r < NV; r++) {
-      double a = ::fabs(A[r * NV + p]);
-      if (a > best) { best = a; piv = r; }
-    }
-    if (piv != p) {
-#pragma unroll
-      for (int c = 0; c < NV; c++) {
-        double t = A[p * NV + c]; A[p * NV + c] = A[piv * NV + c]; A[piv * NV + c] = t;
-        t = Inv[p * NV + c]; Inv[p * NV + c] = Inv[piv * NV + c]; Inv[piv * NV + c] = t;
-      }
-    }
-    const double inv = 1.0 / A[p * NV + p];
-#pragma unroll
-    for (int c = 0; c < NV; c++) { A[p * NV + c] *= inv; Inv[p * NV + c] *= inv; }
-#pragma unroll
-    for (int r = 0; r < NV; r++) {
-      if (r == p) continue;
-      const double f = A[r * NV + p];
-#pragma unroll
-      for (int c = 0; c < NV; c++) {
-        A[r * NV + c] -= f * A[p * NV + c];
-        Inv[r * NV + c] -= f * Inv[p * NV + c];
-      }
-    }
-  }
-}
-
-template <typename TM, int NB>
-__device__ __forceinline__ void load_block(const TM *__restrict__ base, int slot, int N, int i, double *B) {
-#pragma unroll
-  for (int e = 0; e < NB; e++) B[e] = (double)base[((size_t)slot * NB + e) * N + i];
-}
-template <typename TM, int NB>
-__device__ __forceinline__ void store_block(TM *__restrict__ base, int slot, int N, int i, const double *B) {
-#pragma unroll
-  for (int e = 0; e < NB; e++) base[((size_t)slot * NB + e) * N + i] = (TM)B[e];
-}
-
 // block Jacobi: LU slot 0 <- inverse of the diagonal block
 template <int ND, typename TM>
 __global__ __launch_bounds__(256) void k_bj_setup(const TM *__restrict__ A, TM *__restrict__ LU, int N) {
@@ -371,44 +280,6 @@ __global__ __launch_bounds__(256) void k_ilu_factor_color(TM *__restrict__ LU, D
 // ---------------------------------------------------------------------------
 // Krylov kernels (vectors [N][nv], type TV)
 // ---------------------------------------------------------------------------
-template <int ND, typename TM, typename TV>
-__global__ __launch_bounds__(256) void k_spmv(const TM *__restrict__ A, DevMesh dm, const TV *__restrict__ x,
-                                              TV *__restrict__ y) {
-  constexpr int NV = ND + 2, K = ND + 1, NB = NV * NV;
-  const int N = dm.N;
-  const int i = cell_index();
-  if (i >= N) return;
-  double acc[NV];
-  double xi[NV];
-#pragma unroll
-  for (int c = 0; c < NV; c++) xi[c] = (double)x[(size_t)i * NV + c];
-#pragma unroll
-  for (int r = 0; r < NV; r++) {
-    double s = 0;
-#pragma unroll
-    for (int c = 0; c < NV; c++) s += (double)A[((size_t)(r * NV + c)) * N + i] * xi[c];
-    acc[r] = s;
-  }
-#pragma unroll
-  for (int m = 0; m < K; m++) {
-    const int j = dm.nbr[(size_t)m * N + i];
-    if (j < 0) continue;
-    double xj[NV];
-#pragma unroll
-    for (int c = 0; c < NV; c++) xj[c] = (double)x[(size_t)j * NV + c];
-    const TM *Ao = A + (size_t)(1 + m) * NB * N;
-#pragma unroll
-    for (int r = 0; r < NV; r++) {
-      double s = 0;
-#pragma unroll
-      for (int c = 0; c < NV; c++) s += (double)Ao[((size_t)(r * NV + c)) * N + i] * xj[c];
-      acc[r] += s;
-    }
-  }
-#pragma unroll
-  for (int r = 0; r < NV; r++) y[(size_t)i * NV + r] = (TV)acc[r];
-}
-
 // z = inv(D_i) r_i (block Jacobi; also the diagonal step of the U solve)
 template <int ND, typename TM, typename TV>
 __global__ __launch_bounds__(256) void k_bj_apply(const TM *__restrict__ LU, int N, const TV *__restrict__ r,
@@ -699,8 +570,9 @@ static int impl_alloc(bes_ctx *c, Impl *im) {
 }
 
 static void impl_release(Impl *im) {
+  lp::lp_release(im);
   void *ptrs[] = {im->fd.fo, im->fd.fn, im->fd.fnrm, im->fd.cf, im->fd.mrev, im->color, im->crow, im->JF,
-                  im->b, im->x, im->A, im->LU, im->V, im->Z, im->part, im->dots, im->bits};
+                  im->b, im->x, im->A, im->LU, im->LM, im->V, im->Z, im->part, im->dots, im->bits};
   for (void *p : ptrs)
     if (p) hipFree(p);
   delete im;
@@ -751,6 +623,7 @@ static int factor(bes_ctx *c, Impl *im) {
   const size_t mat = (size_t)(c->k + 1) * nv * nv * N;
   const double used = (double)(N + 2 * im->ninternal) * nv * nv * sizeof(TM);   // stored blocks
   ProfScope ps(c, kProfFactor, im->precond == 1 ? 2.0 * N * nv * nv * sizeof(TM) : 3.0 * used);
+  if (im->precond == 3) return lp::lp_build(c, im);
   if (im->precond == 1) {
     if (c->nd == 2) hipLaunchKernelGGL((k_bj_setup<2, TM>), dim3(grid_for(N, bs)), dim3(bs), 0, c->stream, (const TM *)im->A, (TM *)im->LU, N);
     else hipLaunchKernelGGL((k_bj_setup<3, TM>), dim3(grid_for(N, bs)), dim3(bs), 0, c->stream, (const TM *)im->A, (TM *)im->LU, N);
@@ -781,9 +654,13 @@ static int precond_apply(bes_ctx *c, Impl *im, const TV *r, TV *z, TV *tmp) {
   // neighbour ids and colours; Jacobi: diagonal inverse, r, z
   ProfScope ps(c, kProfPrecond,
                im->precond == 2 ? blocks * sizeof(TM) + 4.0 * vb + 2.0 * N * (4.0 * c->k + 4.0)
-                                : (im->precond == 1 ? (double)N * c->nv * c->nv * sizeof(TM) + 2.0 * vb : 2.0 * vb));
+               : im->precond == 3 ? blocks * sizeof(TM) + 2.0 * vb + 4.0 * N * c->k
+               : (im->precond == 1 ? (double)N * c->nv * c->nv * sizeof(TM) + 2.0 * vb : 2.0 * vb));
   if (im->precond == 0) {
     hipLaunchKernelGGL((k_scale<TV, TV>), dim3(kRedBlocks), dim3(256), 0, c->stream, r, 1.0, z, n);
+  } else if (im->precond == 3) {   // learned operator: one block SpMV
+    if (c->nd == 2) hipLaunchKernelGGL((k_spmv<2, TM, TV>), dim3(grid_for(N, bs)), dim3(bs), 0, c->stream, (const TM *)im->LM, c->dm, r, z);
+    else hipLaunchKernelGGL((k_spmv<3, TM, TV>), dim3(grid_for(N, bs)), dim3(bs), 0, c->stream, (const TM *)im->LM, c->dm, r, z);
   } else if (im->precond == 1) {
     if (c->nd == 2) hipLaunchKernelGGL((k_bj_apply<2, TM, TV>), dim3(grid_for(N, bs)), dim3(bs), 0, c->stream, (const TM *)im->LU, N, r, z);
     else hipLaunchKernelGGL((k_bj_apply<3, TM, TV>), dim3(grid_for(N, bs)), dim3(bs), 0, c->stream, (const TM *)im->LU, N, r, z);
@@ -996,9 +873,9 @@ void bes_implicit_free(bes_ctx *c) {
 }
 
 int bes_implicit_setup(bes_ctx *c, int precision, int precond, int restart, int maxiter, double rtol) {
-  if (!c || (precision != 64 && precision != 32) || precond < 0 || precond > 2 || restart < 1 || restart > 30 ||
+  if (!c || (precision != 64 && precision != 32) || precond < 0 || precond > 3 || restart < 1 || restart > 30 ||
       maxiter < 1 || !(rtol > 0)) {
-    set_error("invalid implicit setup argument (precision 64/32, precond 0-2, restart 1-30)");
+    set_error("invalid implicit setup argument (precision 64/32, precond 0-3, restart 1-30)");
     return BES_ERR_ARG;
   }
   if (c->B != 1) {
@@ -1015,6 +892,14 @@ int bes_implicit_setup(bes_ctx *c, int precision, int precond, int restart, int 
   c->impl = im;
   int s = impl_alloc(c, im);
   if (s) { bes_implicit_free(c); return s; }
+  return BES_OK;
+}
+
+int bes_implicit_set_precond(bes_ctx *c, int precond) {
+  if (!c || !c->impl || precond < 0 || precond > 3) { set_error("call bes_implicit_setup first / precond 0-3"); return BES_ERR_ARG; }
+  Impl *im = impl_of(c);
+  im->precond = precond;
+  im->factored = false;
   return BES_OK;
 }
 
@@ -1141,22 +1026,38 @@ long bes_implicit_nnz(bes_ctx *c) {
   return nz * c->nv * c->nv;
 }
 
-int bes_linear_solve(bes_ctx *c, int *iters, double *relres) {
-  if (!c || !c->impl || !impl_of(c)->assembled) { set_error("system not assembled"); return BES_ERR_ARG; }
-  Impl *im = impl_of(c);
+static int solve_once(bes_ctx *c, Impl *im, int *it, double *rr) {
   int s;
-  int it = 0;
-  double rr = 0;
   if (im->prec == 64) {
     if (!im->factored && (s = factor<double>(c, im))) return s;
     im->factored = true;
-    s = gmres<double, double>(c, im, &it, &rr);
-  } else {
-    if (!im->factored && (s = factor<float>(c, im))) return s;
-    im->factored = true;
-    s = gmres<float, float>(c, im, &it, &rr);
+    return gmres<double, double>(c, im, it, rr);
   }
-  if (s) return s;
+  if (!im->factored && (s = factor<float>(c, im))) return s;
+  im->factored = true;
+  return gmres<float, float>(c, im, it, rr);
+}
+
+int bes_linear_solve(bes_ctx *c, int *iters, double *relres) {
+  if (!c || !c->impl || !impl_of(c)->assembled) { set_error("system not assembled"); return BES_ERR_ARG; }
+  Impl *im = impl_of(c);
+  if (im->precond == 3 && !im->lp) { set_error("learned preconditioner not initialised (bes_lp_init)"); return BES_ERR_ARG; }
+  int s;
+  int it = 0;
+  double rr = 0;
+  if ((s = solve_once(c, im, &it, &rr))) return s;
+  if (im->precond == 3 && !(rr <= im->rtol * 1.0001) && !(rr == 0)) {
+    // the learned operator stalled: solve again with multicolour ILU(0)
+    int it2 = 0;
+    im->precond = 2;
+    im->factored = false;
+    s = solve_once(c, im, &it2, &rr);
+    im->precond = 3;
+    im->factored = false;
+    im->lp_fallbacks++;
+    it += it2;
+    if (s) return s;
+  }
   if ((s = prof_collect(c))) return s;
   im->last_iters = it;
   im->last_relres = rr;

@@ -1,0 +1,155 @@
+// Implicit-path workspace and the block kernels shared by implicit.hip and
+// learned.hip.  Matrix layout ("ELL-BSR"): slot 0 = diagonal block, slot 1+m =
+// the block of the neighbour through local face m; entries stored [slot][e][N].
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "context.hpp"
+#include "kernels_common.hpp"
+
+namespace bes {
+
+constexpr int kRedBlocks = 1024;   // partial-sum blocks of the reductions
+
+struct FaceDev {
+  int F = 0;
+  int *fo = nullptr;       // [F] owner (device order)
+  int *fn = nullptr;       // [F] neighbour, -1 slip wall, -2 far field
+  double *fnrm = nullptr;  // [nd][F] unit normal
+  int *cf = nullptr;       // [k][N] face id of (cell, local face)
+  int *mrev = nullptr;     // [k][N] local index of this face in the neighbour
+};
+
+template <typename TM>
+struct Mat {
+  TM *val = nullptr;       // [(k+1)*nv*nv][N]
+};
+
+struct Impl {
+  int prec = 64;           // 64 or 32
+  int precond = 2;         // 0 none, 1 block Jacobi, 2 multicolour ILU(0), 3 learned
+  int restart = 30, maxiter = 1000;
+  double rtol = 1e-7;
+  int repeat_nv = 1;
+  FaceDev fd;
+  double *JF = nullptr;    // [F][2][nv*nv]
+  double *b = nullptr;     // [N][nv] FP64 right-hand side
+  double *x = nullptr;     // [N][nv] FP64 solution
+  void *A = nullptr;       // matrix values (TM)
+  void *LU = nullptr;      // factor / inverse diagonal blocks (TM), [(k+1)*nv*nv][N]
+  void *LM = nullptr;      // learned operator blocks (TM), [(k+1)*nv*nv][N] (learned.hip)
+  void *lp = nullptr;      // learned-preconditioner network state (learned.hip)
+  int lp_fallbacks = 0;    // solves that fell back to ILU(0)
+  void *V = nullptr;       // Krylov basis (TV) [(m+1)][N*nv]
+  void *Z = nullptr;       // work vectors (TV) [3][N*nv]
+  double *part = nullptr;  // reduction partials [32][kRedBlocks]
+  double *dots = nullptr;  // [64]
+  unsigned long long *bits = nullptr;   // [16] Linf monitors
+  int ncolor = 0;
+  std::vector<int> color_start;   // host: rows of colour c are crow[color_start[c] .. color_start[c+1])
+  int *crow = nullptr;            // [N] rows grouped by colour
+  int *color = nullptr;           // [N]
+  long ninternal = 0;              // internal faces (off-diagonal block pairs)
+  int last_iters = 0;
+  double last_relres = 0;
+  bool assembled = false, factored = false;
+};
+
+// ---------------------------------------------------------------------------
+// small dense block helpers (NV x NV, row-major, registers)
+// ---------------------------------------------------------------------------
+template <int NV>
+__device__ __forceinline__ void block_inverse(double *A, double *Inv) {
+  // Gauss-Jordan with partial pivoting
+#pragma unroll
+  for (int r = 0; r < NV; r++)
+#pragma unroll
+    for (int c = 0; c < NV; c++) Inv[r * NV + c] = (r == c) ? 1.0 : 0.0;
+#pragma unroll
+  for (int p = 0; p < NV; p++) {
+    int piv = p;
+    double best = ::fabs(A[p * NV + p]);
+#pragma unroll
+    for (int r = p + 1; r < NV; r++) {
+      double a = ::fabs(A[r * NV + p]);
+      if (a > best) { best = a; piv = r; }
+    }
+    if (piv != p) {
+#pragma unroll
+      for (int c = 0; c < NV; c++) {
+        double t = A[p * NV + c]; A[p * NV + c] = A[piv * NV + c]; A[piv * NV + c] = t;
+        t = Inv[p * NV + c]; Inv[p * NV + c] = Inv[piv * NV + c]; Inv[piv * NV + c] = t;
+      }
+    }
+    const double inv = 1.0 / A[p * NV + p];
+#pragma unroll
+    for (int c = 0; c < NV; c++) { A[p * NV + c] *= inv; Inv[p * NV + c] *= inv; }
+#pragma unroll
+    for (int r = 0; r < NV; r++) {
+      if (r == p) continue;
+      const double f = A[r * NV + p];
+#pragma unroll
+      for (int c = 0; c < NV; c++) {
+        A[r * NV + c] -= f * A[p * NV + c];
+        Inv[r * NV + c] -= f * Inv[p * NV + c];
+      }
+    }
+  }
+}
+
+template <typename TM, int NB>
+__device__ __forceinline__ void load_block(const TM *__restrict__ base, int slot, int N, int i, double *B) {
+#pragma unroll
+  for (int e = 0; e < NB; e++) B[e] = (double)base[((size_t)slot * NB + e) * N + i];
+}
+template <typename TM, int NB>
+__device__ __forceinline__ void store_block(TM *__restrict__ base, int slot, int N, int i, const double *B) {
+#pragma unroll
+  for (int e = 0; e < NB; e++) base[((size_t)slot * NB + e) * N + i] = (TM)B[e];
+}
+
+// ---------------------------------------------------------------------------
+// y = A x for an ELL-BSR matrix (vectors [N][nv], type TV)
+// ---------------------------------------------------------------------------
+template <int ND, typename TM, typename TV>
+__global__ __launch_bounds__(256) void k_spmv(const TM *__restrict__ A, DevMesh dm, const TV *__restrict__ x,
+                                              TV *__restrict__ y) {
+  constexpr int NV = ND + 2, K = ND + 1, NB = NV * NV;
+  const int N = dm.N;
+  const int i = cell_index();
+  if (i >= N) return;
+  double acc[NV];
+  double xi[NV];
+#pragma unroll
+  for (int c = 0; c < NV; c++) xi[c] = (double)x[(size_t)i * NV + c];
+#pragma unroll
+  for (int r = 0; r < NV; r++) {
+    double s = 0;
+#pragma unroll
+    for (int c = 0; c < NV; c++) s += (double)A[((size_t)(r * NV + c)) * N + i] * xi[c];
+    acc[r] = s;
+  }
+#pragma unroll
+  for (int m = 0; m < K; m++) {
+    const int j = dm.nbr[(size_t)m * N + i];
+    if (j < 0) continue;
+    double xj[NV];
+#pragma unroll
+    for (int c = 0; c < NV; c++) xj[c] = (double)x[(size_t)j * NV + c];
+    const TM *Ao = A + (size_t)(1 + m) * NB * N;
+#pragma unroll
+    for (int r = 0; r < NV; r++) {
+      double s = 0;
+#pragma unroll
+      for (int c = 0; c < NV; c++) s += (double)Ao[((size_t)(r * NV + c)) * N + i] * xj[c];
+      acc[r] += s;
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < NV; r++) y[(size_t)i * NV + r] = (TV)acc[r];
+}
+
+}  // namespace bes

@@ -145,11 +145,15 @@ int bes_jacobian_batch(int ndim, int kind, long n, const double *UL, const doubl
  * bes_implicit_setup: allocate the matrix and solver workspace.
  *   precision 64 (FP64 matrix, factors and Krylov vectors) or 32 (FP32 inner
  *   solve; the right-hand side and the update stay FP64); precond 0 none,
- *   1 block Jacobi, 2 block ILU(0) in multicolour order; GMRES(restart),
+ *   1 block Jacobi, 2 block ILU(0) in multicolour order, 3 the learned
+ *   operator (bes_lp_*, below; falls back to ILU(0) for a solve it stalls);
+ *   GMRES(restart),
  *   restart <= 30, stops at ||b - A x|| <= rtol ||b|| (the reference's
  *   LINEAR_SOLVER_ABS_TOL is this relative tolerance: CoefficientMatrix.cpp:304).
  *   Replaces CoefficientMatrix::initialize (CoefficientMatrix.cpp:114-164). */
 int bes_implicit_setup(bes_ctx *ctx, int precision, int precond, int restart, int maxiter, double rtol);
+/* switch the preconditioner (0-3) of an existing setup; the next solve factors again */
+int bes_implicit_set_precond(bes_ctx *ctx, int precond);
 /* time step, gradient, limiter, face Jacobians and the system A x = b at the
  * current state (SolverImplicit.cpp:307-320 = resetSystemOfEquation_ + dt +
  * gradient + limiter + computeFaceFluxes_).  repeat_nv != 0 reproduces the
@@ -182,6 +186,37 @@ int bes_implicit_update(bes_ctx *ctx);
 int bes_implicit_run(bes_ctx *ctx, int nsteps, double cfl_min, double cfl_max, const double *exponents,
                      int local, int repeat_nv, double stop_drop, double *hist, int *steps_done);
 
+/* ---- learned preconditioner (no reference counterpart; an alternative to the
+ *      AMGCL preconditioner of CoefficientMatrix.cpp:210-325, used with
+ *      precond 3).  A message-passing network on the cell dual graph (width 32,
+ *      3 layers, BF16 matrix-core kernels with FP32 accumulation) picks, per
+ *      face and row, a blend G in (0,1) between block Jacobi and the
+ *      first-order Neumann approximation of A^-1:
+ *        (M v)_i = D_i^-1 v_i - sum_m G_im D_i^-1 O_im D_j^-1 v_j.
+ *      It is trained on the GPU on the assembled system by Adam on the loss
+ *      mean_q ||r_K||^2/||v_q||^2, r_{k+1} = r_k - A M r_k, r_0 = v_q (probe
+ *      q = 0 the right-hand side, the rest seeded Gaussian vectors).
+ * bes_lp_init: network state after bes_implicit_setup; deterministic weights
+ *   from seed, zero decoder weights with bias 3 (G = 0.953 everywhere).
+ * bes_lp_set_params / bes_lp_get_params: the flat FP32 parameter vector
+ *   (n = bes_lp_param_count()).
+ * bes_lp_operator: builds M for the assembled system; Md [ncell][nv][nv],
+ *   Me [ncell][k][nv][nv] (by local face), G [ncell][k][nv]; any may be NULL.
+ * bes_lp_loss_grad: loss and gradient for caller-given probe vectors
+ *   [nprobe][ncell][nv] (reference cell order), nrich Richardson steps.
+ * bes_lp_train: steps Adam steps (learning rate lr) on the assembled system;
+ *   loss_hist [steps] may be NULL.
+ * bes_lp_fallbacks: solves so far that fell back to ILU(0). ---- */
+int bes_lp_param_count(void);
+int bes_lp_init(bes_ctx *ctx, unsigned long long seed);
+int bes_lp_set_params(bes_ctx *ctx, const float *params, int n);
+int bes_lp_get_params(bes_ctx *ctx, float *params, int n);
+int bes_lp_operator(bes_ctx *ctx, double *Md, double *Me, double *G);
+int bes_lp_loss_grad(bes_ctx *ctx, const double *probes, int nprobe, int nrich, double *loss, float *grad);
+int bes_lp_train(bes_ctx *ctx, int steps, double lr, int nprobe, int nrich, unsigned long long seed,
+                 double *loss_hist);
+int bes_lp_fallbacks(bes_ctx *ctx);
+
 /* ---- measurement (no counterpart in the reference, which times nothing):
  *      when profiling is on, every kernel launch is bracketed by HIP events on
  *      the context's stream; bes_get_profile returns, per kernel kind, the
@@ -190,7 +225,8 @@ int bes_implicit_run(bes_ctx *ctx, int nsteps, double cfl_min, double cfl_max, c
  *      gathers served by cache not counted) since profiling was switched on.
  *      kinds: 0 time step, 1 gradient+limiter, 2 residual+update,
  *      3 Jacobian, 4 assembly, 5 SpMV, 6 preconditioner apply, 7 Krylov vector
- *      ops, 8 factorisation, 9 implicit update ---- */
+ *      ops, 8 factorisation, 9 implicit update, 10 face flux, 11 learned
+ *      preconditioner (build and training) ---- */
 int bes_set_profiling(bes_ctx *ctx, int on);
 int bes_get_profile(bes_ctx *ctx, double *total_ms, long *launches, double *bytes, int nkinds);
 int bes_synchronize(bes_ctx *ctx);
