@@ -78,6 +78,9 @@ OPTIONAL_SIGS = {
     "bes_get_profile": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int],
     "bes_synchronize": [C.c_void_p],
     "bes_implicit_set_precond": [C.c_void_p, C.c_int],
+    "bes_implicit_set_reorth": [C.c_void_p, C.c_double],
+    "bes_implicit_stats": [C.c_void_p, C.c_void_p, C.c_void_p],
+    "bes_implicit_continue": [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p],
     "bes_set_batch": [C.c_void_p, C.c_int],
     "bes_get_batch": [C.c_void_p],
     "bes_set_freestream_batch": [C.c_void_p, C.c_void_p],
@@ -272,6 +275,15 @@ class Context:
     def implicit_setup(self, precision=64, precond="ilu0", restart=30, maxiter=1000, rtol=1e-7):
         _check(lib().bes_implicit_setup(self.h, precision, self.PRECOND[precond], restart, maxiter, rtol))
 
+    def implicit_set_reorth(self, eta):
+        _check(lib().bes_implicit_set_reorth(self.h, eta))
+
+    def implicit_stats(self):
+        r = C.c_long()
+        f = C.c_int()
+        _check(lib().bes_implicit_stats(self.h, C.byref(r), C.byref(f)))
+        return {"reorths": r.value, "lp_fallbacks": f.value}
+
     def implicit_set_precond(self, precond):
         _check(lib().bes_implicit_set_precond(self.h, self.PRECOND[precond]))
 
@@ -354,6 +366,14 @@ class Context:
 
     def lp_fallbacks(self):
         return lib().bes_lp_fallbacks(self.h)
+
+    def implicit_continue(self, nsteps):
+        """nsteps more steps of the last implicit_run; returns (history, reached)."""
+        hist = np.zeros((nsteps, 2 * self.nv + 3))
+        done = C.c_int()
+        reached = C.c_int()
+        _check(lib().bes_implicit_continue(self.h, nsteps, _p(hist), C.byref(done), C.byref(reached)), self.h)
+        return hist[:done.value], bool(reached.value)
 
     def explicit_graph(self, on=True):
         """Replay each pseudo-time step as one captured HIP graph."""

@@ -188,59 +188,70 @@ __global__ __launch_bounds__(256) void k_bj_setup(const TM *__restrict__ A, TM *
 // block ILU(0) factorisation of the rows of one colour (all lower colours done).
 // LU holds a copy of A on entry; on exit slot 0 holds inv(U_ii), lower slots L_ik,
 // upper slots U_ij.
+// colour-major copy of the matrix: LU[slot][e][p] = A[slot][e][crow[p]]
+template <typename TM>
+__global__ void k_ilu_gather(const TM *__restrict__ A, const int *__restrict__ crow, int N, int nslot_e,
+                             TM *__restrict__ LU) {
+  const size_t tot = (size_t)nslot_e * N;
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < tot; q += (size_t)gridDim.x * blockDim.x) {
+    const size_t se = q / N, p = q - se * N;
+    LU[q] = A[se * N + crow[p]];
+  }
+}
+
+// block ILU(0) of the rows at positions p0 .. p0+nrows-1 (one colour); rows of
+// earlier colours are final.  Elimination order: colour, then position.
 template <int ND, typename TM>
-__global__ __launch_bounds__(256) void k_ilu_factor_color(TM *__restrict__ LU, DevMesh dm, FaceDev fd,
-                                                          const int *__restrict__ color,
-                                                          const int *__restrict__ rows, int nrows) {
+__global__ __launch_bounds__(256) void k_ilu_factor_color(TM *__restrict__ LU, int N, const int *__restrict__ crow,
+                                                          const int *__restrict__ cnb, const int *__restrict__ cnp,
+                                                          const unsigned char *__restrict__ clow, int p0, int nrows) {
   constexpr int NV = ND + 2, K = ND + 1, NB = NV * NV;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nrows) return;
-  const int N = dm.N;
-  const int i = rows[t];
-  const int ci = color[i];
-  int nb[K], cn[K];
+  const int p = p0 + t;
+  const int i = crow[p];
+  const unsigned lowm = clow[p];
+  int nb[K], np_[K];
 #pragma unroll
   for (int m = 0; m < K; m++) {
-    nb[m] = dm.nbr[(size_t)m * N + i];
-    cn[m] = nb[m] >= 0 ? color[nb[m]] : 0x7fffffff;
+    nb[m] = cnb[(size_t)m * N + p];
+    np_[m] = cnp[(size_t)m * N + p];
   }
-  // lower neighbours in elimination order (colour, then row index)
+  // lower neighbours in elimination order (position order)
   int ord[K];
   int nl = 0;
 #pragma unroll
   for (int m = 0; m < K; m++)
-    if (nb[m] >= 0 && cn[m] < ci) ord[nl++] = m;
+    if ((lowm >> m) & 1) ord[nl++] = m;
   for (int a = 1; a < nl; a++)
     for (int bq = a; bq > 0; bq--) {
       const int x = ord[bq - 1], y = ord[bq];
-      const bool swap = cn[x] > cn[y] || (cn[x] == cn[y] && nb[x] > nb[y]);
-      if (swap) { ord[bq - 1] = y; ord[bq] = x; }
+      if (np_[x] > np_[y]) { ord[bq - 1] = y; ord[bq] = x; }
     }
   double Aii[NB];
-  load_block<TM, NB>(LU, 0, N, i, Aii);
+  load_block<TM, NB>(LU, 0, N, p, Aii);
   for (int q = 0; q < nl; q++) {
     const int m = ord[q];
-    const int k = nb[m];
+    const int pk = np_[m];
     double Lik[NB], Uinv[NB], Aik[NB];
-    load_block<TM, NB>(LU, 1 + m, N, i, Aik);
-    load_block<TM, NB>(LU, 0, N, k, Uinv);   // row k is final: slot 0 = inv(U_kk)
+    load_block<TM, NB>(LU, 1 + m, N, p, Aik);
+    load_block<TM, NB>(LU, 0, N, pk, Uinv);   // row k is final: slot 0 = inv(U_kk)
 #pragma unroll
     for (int r = 0; r < NV; r++)
 #pragma unroll
       for (int c = 0; c < NV; c++) {
         double s = 0;
 #pragma unroll
-        for (int p = 0; p < NV; p++) s += Aik[r * NV + p] * Uinv[p * NV + c];
+        for (int pp = 0; pp < NV; pp++) s += Aik[r * NV + pp] * Uinv[pp * NV + c];
         Lik[r * NV + c] = s;
       }
-    store_block<TM, NB>(LU, 1 + m, N, i, Lik);
+    store_block<TM, NB>(LU, 1 + m, N, p, Lik);
     // A_ij -= L_ik U_kj for j in row k's upper part that are in row i's pattern
+    const unsigned lowk = clow[pk];
 #pragma unroll
     for (int mk = 0; mk < K; mk++) {
-      const int jk = dm.nbr[(size_t)mk * N + k];
-      if (jk < 0) continue;
-      const int cj = color[jk];
-      if (!(cj > cn[m] || (cj == cn[m] && jk > k))) continue;   // not upper in row k
+      const int jk = cnb[(size_t)mk * N + pk];
+      if (jk < 0 || ((lowk >> mk) & 1)) continue;   // not upper in row k
       int target = -1;   // -2 = diagonal of row i
       if (jk == i) target = -2;
       else {
@@ -249,7 +260,7 @@ __global__ __launch_bounds__(256) void k_ilu_factor_color(TM *__restrict__ LU, D
       }
       if (target == -1) continue;
       double Ukj[NB];
-      load_block<TM, NB>(LU, 1 + mk, N, k, Ukj);
+      load_block<TM, NB>(LU, 1 + mk, N, pk, Ukj);
       double upd[NB];
 #pragma unroll
       for (int r = 0; r < NV; r++)
@@ -257,7 +268,7 @@ __global__ __launch_bounds__(256) void k_ilu_factor_color(TM *__restrict__ LU, D
         for (int c = 0; c < NV; c++) {
           double s = 0;
 #pragma unroll
-          for (int p = 0; p < NV; p++) s += Lik[r * NV + p] * Ukj[p * NV + c];
+          for (int pp = 0; pp < NV; pp++) s += Lik[r * NV + pp] * Ukj[pp * NV + c];
           upd[r * NV + c] = s;
         }
       if (target == -2) {
@@ -265,16 +276,89 @@ __global__ __launch_bounds__(256) void k_ilu_factor_color(TM *__restrict__ LU, D
         for (int e = 0; e < NB; e++) Aii[e] -= upd[e];
       } else {
         double Aij[NB];
-        load_block<TM, NB>(LU, 1 + target, N, i, Aij);
+        load_block<TM, NB>(LU, 1 + target, N, p, Aij);
 #pragma unroll
         for (int e = 0; e < NB; e++) Aij[e] -= upd[e];
-        store_block<TM, NB>(LU, 1 + target, N, i, Aij);
+        store_block<TM, NB>(LU, 1 + target, N, p, Aij);
       }
     }
   }
   double Inv[NB];
   block_inverse<NV>(Aii, Inv);
-  store_block<TM, NB>(LU, 0, N, i, Inv);
+  store_block<TM, NB>(LU, 0, N, p, Inv);
+}
+
+// forward sweep of one colour: y_i = r_i - sum_(k lower) L_ik y_k
+template <int ND, typename TM, typename TV>
+__global__ __launch_bounds__(256) void k_ilu_lower(const TM *__restrict__ LU, int N, const int *__restrict__ crow,
+                                                   const int *__restrict__ cnb, const unsigned char *__restrict__ clow,
+                                                   int p0, int nrows, const TV *__restrict__ r, TV *__restrict__ y) {
+  constexpr int NV = ND + 2, K = ND + 1, NB = NV * NV;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nrows) return;
+  const int p = p0 + t;
+  const int i = crow[p];
+  const unsigned lowm = clow[p];
+  double acc[NV];
+#pragma unroll
+  for (int c = 0; c < NV; c++) acc[c] = (double)r[(size_t)i * NV + c];
+#pragma unroll
+  for (int m = 0; m < K; m++) {
+    if (!((lowm >> m) & 1)) continue;
+    const int k = cnb[(size_t)m * N + p];
+    double yk[NV];
+#pragma unroll
+    for (int c = 0; c < NV; c++) yk[c] = (double)y[(size_t)k * NV + c];
+    const TM *L = LU + (size_t)(1 + m) * NB * N;
+#pragma unroll
+    for (int a = 0; a < NV; a++) {
+      double s = 0;
+#pragma unroll
+      for (int c = 0; c < NV; c++) s += (double)L[((size_t)(a * NV + c)) * N + p] * yk[c];
+      acc[a] -= s;
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < NV; c++) y[(size_t)i * NV + c] = (TV)acc[c];
+}
+
+// backward sweep of one colour: x_i = inv(U_ii) (y_i - sum_(j upper) U_ij x_j)
+template <int ND, typename TM, typename TV>
+__global__ __launch_bounds__(256) void k_ilu_upper(const TM *__restrict__ LU, int N, const int *__restrict__ crow,
+                                                   const int *__restrict__ cnb, const unsigned char *__restrict__ clow,
+                                                   int p0, int nrows, const TV *__restrict__ y, TV *__restrict__ x) {
+  constexpr int NV = ND + 2, K = ND + 1, NB = NV * NV;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nrows) return;
+  const int p = p0 + t;
+  const int i = crow[p];
+  const unsigned lowm = clow[p];
+  double acc[NV];
+#pragma unroll
+  for (int c = 0; c < NV; c++) acc[c] = (double)y[(size_t)i * NV + c];
+#pragma unroll
+  for (int m = 0; m < K; m++) {
+    const int j = cnb[(size_t)m * N + p];
+    if (j < 0 || ((lowm >> m) & 1)) continue;
+    double xj[NV];
+#pragma unroll
+    for (int c = 0; c < NV; c++) xj[c] = (double)x[(size_t)j * NV + c];
+    const TM *Uo = LU + (size_t)(1 + m) * NB * N;
+#pragma unroll
+    for (int a = 0; a < NV; a++) {
+      double s = 0;
+#pragma unroll
+      for (int c = 0; c < NV; c++) s += (double)Uo[((size_t)(a * NV + c)) * N + p] * xj[c];
+      acc[a] -= s;
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < NV; a++) {
+    double s = 0;
+#pragma unroll
+    for (int c = 0; c < NV; c++) s += (double)LU[((size_t)(a * NV + c)) * N + p] * acc[c];
+    x[(size_t)i * NV + a] = (TV)s;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -373,28 +457,107 @@ __global__ __launch_bounds__(256) void k_ilu_upper(const TM *__restrict__ LU, De
 }
 
 // partial dot products of w with up to 8 basis vectors V[i0 .. i0+cnt), plus ||w||^2
+// 16-byte packets of Krylov vectors (4 floats or 2 doubles); vectors start on
+// 64-element boundaries (Impl::vstride), so packet loads are aligned
+template <typename T>
+struct Pk {
+  static constexpr int W = 16 / sizeof(T);
+  T v[16 / sizeof(T)];
+};
+template <typename T>
+__device__ __forceinline__ Pk<T> ldp(const T *p) {
+  Pk<T> r;
+  const uint4 u = *reinterpret_cast<const uint4 *>(p);
+  __builtin_memcpy(r.v, &u, 16);
+  return r;
+}
+template <typename T>
+__device__ __forceinline__ void stp(T *p, const Pk<T> &r) {
+  uint4 u;
+  __builtin_memcpy(&u, r.v, 16);
+  *reinterpret_cast<uint4 *>(p) = u;
+}
+
+__device__ __forceinline__ void block_store_sums(const double *acc, int nq, double *part) {
+  __shared__ double sh[9][4];
+  for (int q = 0; q < nq; q++) {
+    const double s = wave_sum(acc[q]);
+    if ((threadIdx.x & 63) == 0) sh[q][threadIdx.x >> 6] = s;
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < nq) {
+    const int q = threadIdx.x;
+    part[(size_t)q * gridDim.x + blockIdx.x] = sh[q][0] + sh[q][1] + sh[q][2] + sh[q][3];
+  }
+}
+
+// up to 8 dot products <V_(i0+q), w> and ||w||^2, per-block partials
 template <typename TV>
 __global__ __launch_bounds__(256) void k_mdot(const TV *__restrict__ V, size_t stride, int i0, int cnt,
                                               const TV *__restrict__ w, size_t n, double *__restrict__ part) {
+  constexpr int W = Pk<TV>::W;
   double acc[9];
 #pragma unroll
   for (int q = 0; q < 9; q++) acc[q] = 0;
-  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
-    const double we = (double)w[e];
+  const size_t np = n / W, tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x, nt = (size_t)gridDim.x * blockDim.x;
+  for (size_t e = tid; e < np; e += nt) {
+    const Pk<TV> wv = ldp(w + e * W);
 #pragma unroll
     for (int q = 0; q < 8; q++)
-      if (q < cnt) acc[q] += (double)V[(size_t)(i0 + q) * stride + e] * we;
+      if (q < cnt) {
+        const Pk<TV> vv = ldp(V + (size_t)(i0 + q) * stride + e * W);
+#pragma unroll
+        for (int t = 0; t < W; t++) acc[q] += (double)vv.v[t] * (double)wv.v[t];
+      }
+#pragma unroll
+    for (int t = 0; t < W; t++) acc[8] += (double)wv.v[t] * (double)wv.v[t];
+  }
+  for (size_t e = np * W + tid; e < n; e += nt) {   // tail
+    const double we = (double)w[e];
+    for (int q = 0; q < cnt; q++) acc[q] += (double)V[(size_t)(i0 + q) * stride + e] * we;
     acc[8] += we * we;
   }
+  block_store_sums(acc, 9, part);
+}
+
+// w -= sum_q h_q V_q (h read from the multidot chunks: h_q = dots[9*(q/8) + q%8]) and ||w||^2
+template <typename TV>
+__global__ __launch_bounds__(256) void k_orth_update(const TV *__restrict__ V, size_t stride, int cnt,
+                                                     const double *__restrict__ dots, TV *__restrict__ w, size_t n,
+                                                     double *__restrict__ part) {
+  constexpr int W = Pk<TV>::W;
+  __shared__ double hs[32];
+  if ((int)threadIdx.x < cnt) hs[threadIdx.x] = dots[9 * (threadIdx.x >> 3) + (threadIdx.x & 7)];
+  __syncthreads();
+  double nrm = 0;
+  const size_t np = n / W, tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x, nt = (size_t)gridDim.x * blockDim.x;
+  for (size_t e = tid; e < np; e += nt) {
+    Pk<TV> wv = ldp(w + e * W);
+    double we[W];
 #pragma unroll
-  for (int q = 0; q < 9; q++) {
-    double s = wave_sum(acc[q]);
-    __shared__ double sh[9][4];
-    if ((threadIdx.x & 63) == 0) sh[q][threadIdx.x >> 6] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) part[(size_t)q * gridDim.x + blockIdx.x] = sh[q][0] + sh[q][1] + sh[q][2] + sh[q][3];
-    __syncthreads();
+    for (int t = 0; t < W; t++) we[t] = (double)wv.v[t];
+    for (int q = 0; q < cnt; q++) {
+      const Pk<TV> vv = ldp(V + (size_t)q * stride + e * W);
+      const double hq = hs[q];
+#pragma unroll
+      for (int t = 0; t < W; t++) we[t] -= hq * (double)vv.v[t];
+    }
+#pragma unroll
+    for (int t = 0; t < W; t++) {
+      wv.v[t] = (TV)we[t];
+      const double wr = (double)wv.v[t];
+      nrm += wr * wr;
+    }
+    stp(w + e * W, wv);
   }
+  for (size_t e = np * W + tid; e < n; e += nt) {
+    double we = (double)w[e];
+    for (int q = 0; q < cnt; q++) we -= hs[q] * (double)V[(size_t)q * stride + e];
+    w[e] = (TV)we;
+    const double wr = (double)w[e];
+    nrm += wr * wr;
+  }
+  block_store_sums(&nrm, 1, part);
 }
 
 // out[q] = sum over blocks of part[q][.], fixed order (deterministic)
@@ -410,24 +573,18 @@ __global__ void k_reduce_parts(const double *__restrict__ part, int nblk, int nq
   if (threadIdx.x == 0) out[q] = sh[0] + sh[1] + sh[2] + sh[3];
 }
 
-// w -= sum_q h[q] V[q] for q < cnt, returns partial ||w||^2
+// y = a x for two vectors of one type, packets
 template <typename TV>
-__global__ __launch_bounds__(256) void k_orth_update(const TV *__restrict__ V, size_t stride, int cnt,
-                                                     const double *__restrict__ h, TV *__restrict__ w, size_t n,
-                                                     double *__restrict__ part) {
-  double nrm = 0;
-  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
-    double we = (double)w[e];
-    for (int q = 0; q < cnt; q++) we -= h[q] * (double)V[(size_t)q * stride + e];
-    w[e] = (TV)we;
-    const double wr = (double)w[e];
-    nrm += wr * wr;
+__global__ void k_scale_p(const TV *__restrict__ x, double a, TV *__restrict__ y, size_t n) {
+  constexpr int W = Pk<TV>::W;
+  const size_t np = n / W, tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x, nt = (size_t)gridDim.x * blockDim.x;
+  for (size_t e = tid; e < np; e += nt) {
+    Pk<TV> v = ldp(x + e * W);
+#pragma unroll
+    for (int t = 0; t < W; t++) v.v[t] = (TV)(a * (double)v.v[t]);
+    stp(y + e * W, v);
   }
-  double s = wave_sum(nrm);
-  __shared__ double sh[4];
-  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
-  __syncthreads();
-  if (threadIdx.x == 0) part[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
+  for (size_t e = np * W + tid; e < n; e += nt) y[e] = (TV)(a * (double)x[e]);
 }
 
 // y = a*x (scale/copy with conversion)
@@ -538,6 +695,20 @@ static int impl_alloc(bes_ctx *c, Impl *im) {
   }
   im->color_start[ncol] = (int)crow.size();
   im->ncolor = ncol;
+  std::vector<int> cpos(N), cnb((size_t)k * N, -1), cnp((size_t)k * N, -1);
+  std::vector<unsigned char> clow(N, 0);
+  for (int q = 0; q < N; q++) cpos[crow[q]] = q;
+  for (int q = 0; q < N; q++) {
+    const int64_t e = c->perm[crow[q]];
+    for (int m = 0; m < k; m++) {
+      const int64_t j = g.cadj[e * k + m];
+      if (j < 0) continue;
+      const int jd = (int)c->iperm[j];
+      cnb[(size_t)m * N + q] = jd;
+      cnp[(size_t)m * N + q] = cpos[jd];
+      if (cpos[jd] < q) clow[q] |= (unsigned char)(1u << m);
+    }
+  }
   im->fd.F = F;
   im->ninternal = 0;
   for (int f = 0; f < F; f++) if (g.nbr[f] >= 0) im->ninternal++;
@@ -550,7 +721,8 @@ static int impl_alloc(bes_ctx *c, Impl *im) {
   int s;
   if ((s = up(&im->fd.fo, fo)) || (s = up(&im->fd.fn, fn)) || (s = up(&im->fd.fnrm, fnrm)) ||
       (s = up(&im->fd.cf, cf)) || (s = up(&im->fd.mrev, mrev)) || (s = up(&im->color, color)) ||
-      (s = up(&im->crow, crow)))
+      (s = up(&im->crow, crow)) || (s = up(&im->cpos, cpos)) || (s = up(&im->cnb, cnb)) ||
+      (s = up(&im->cnp, cnp)) || (s = up(&im->clow, clow)))
     return s;
   const size_t nvec = (size_t)N * nv;
   const size_t es = im->prec == 32 ? sizeof(float) : sizeof(double);
@@ -561,8 +733,9 @@ static int impl_alloc(bes_ctx *c, Impl *im) {
   BES_HIP(hipMalloc((void **)&im->x, nvec * sizeof(double)));
   BES_HIP(hipMalloc(&im->A, mat * es));
   BES_HIP(hipMalloc(&im->LU, mat * es));
-  BES_HIP(hipMalloc(&im->V, (size_t)(im->restart + 1) * nvec * es));
-  BES_HIP(hipMalloc(&im->Z, 3 * nvec * es));
+  im->vstride = (nvec + 63) / 64 * 64;   // 16-byte aligned packet loads on every vector
+  BES_HIP(hipMalloc(&im->V, (size_t)(im->restart + 1) * im->vstride * es));
+  BES_HIP(hipMalloc(&im->Z, 3 * im->vstride * es));
   BES_HIP(hipMalloc((void **)&im->part, 32 * kRedBlocks * sizeof(double)));
   BES_HIP(hipMalloc((void **)&im->dots, 64 * sizeof(double)));
   BES_HIP(hipMalloc((void **)&im->bits, 16 * sizeof(unsigned long long)));
@@ -572,7 +745,8 @@ static int impl_alloc(bes_ctx *c, Impl *im) {
 static void impl_release(Impl *im) {
   lp::lp_release(im);
   void *ptrs[] = {im->fd.fo, im->fd.fn, im->fd.fnrm, im->fd.cf, im->fd.mrev, im->color, im->crow, im->JF,
-                  im->b, im->x, im->A, im->LU, im->LM, im->V, im->Z, im->part, im->dots, im->bits};
+                  im->b, im->x, im->A, im->LU, im->LM, im->V, im->Z, im->part, im->dots, im->bits,
+                  im->cpos, im->cnb, im->cnp, im->clow};
   for (void *p : ptrs)
     if (p) hipFree(p);
   delete im;
@@ -628,16 +802,17 @@ static int factor(bes_ctx *c, Impl *im) {
     if (c->nd == 2) hipLaunchKernelGGL((k_bj_setup<2, TM>), dim3(grid_for(N, bs)), dim3(bs), 0, c->stream, (const TM *)im->A, (TM *)im->LU, N);
     else hipLaunchKernelGGL((k_bj_setup<3, TM>), dim3(grid_for(N, bs)), dim3(bs), 0, c->stream, (const TM *)im->A, (TM *)im->LU, N);
   } else if (im->precond == 2) {
-    BES_HIP(hipMemcpyAsync(im->LU, im->A, mat * sizeof(TM), hipMemcpyDeviceToDevice, c->stream));
+    hipLaunchKernelGGL((k_ilu_gather<TM>), dim3(2048), dim3(256), 0, c->stream, (const TM *)im->A, im->crow, N,
+                       (c->k + 1) * nv * nv, (TM *)im->LU);
     for (int col = 0; col < im->ncolor; col++) {
       const int r0 = im->color_start[col], nr = im->color_start[col + 1] - r0;
       if (nr == 0) continue;
       if (c->nd == 2)
         hipLaunchKernelGGL((k_ilu_factor_color<2, TM>), dim3(grid_for(nr, bs)), dim3(bs), 0, c->stream, (TM *)im->LU,
-                           c->dm, im->fd, im->color, im->crow + r0, nr);
+                           N, im->crow, im->cnb, im->cnp, im->clow, r0, nr);
       else
         hipLaunchKernelGGL((k_ilu_factor_color<3, TM>), dim3(grid_for(nr, bs)), dim3(bs), 0, c->stream, (TM *)im->LU,
-                           c->dm, im->fd, im->color, im->crow + r0, nr);
+                           N, im->crow, im->cnb, im->cnp, im->clow, r0, nr);
     }
   }
   BES_HIP(hipGetLastError());
@@ -670,20 +845,20 @@ static int precond_apply(bes_ctx *c, Impl *im, const TV *r, TV *z, TV *tmp) {
       if (nr == 0) continue;
       if (c->nd == 2)
         hipLaunchKernelGGL((k_ilu_lower<2, TM, TV>), dim3(grid_for(nr, bs)), dim3(bs), 0, c->stream, (const TM *)im->LU,
-                           c->dm, im->color, im->crow + r0, nr, r, tmp);
+                           N, im->crow, im->cnb, im->clow, r0, nr, r, tmp);
       else
         hipLaunchKernelGGL((k_ilu_lower<3, TM, TV>), dim3(grid_for(nr, bs)), dim3(bs), 0, c->stream, (const TM *)im->LU,
-                           c->dm, im->color, im->crow + r0, nr, r, tmp);
+                           N, im->crow, im->cnb, im->clow, r0, nr, r, tmp);
     }
     for (int col = im->ncolor - 1; col >= 0; col--) {
       const int r0 = im->color_start[col], nr = im->color_start[col + 1] - r0;
       if (nr == 0) continue;
       if (c->nd == 2)
         hipLaunchKernelGGL((k_ilu_upper<2, TM, TV>), dim3(grid_for(nr, bs)), dim3(bs), 0, c->stream, (const TM *)im->LU,
-                           c->dm, im->color, im->crow + r0, nr, tmp, z);
+                           N, im->crow, im->cnb, im->clow, r0, nr, tmp, z);
       else
         hipLaunchKernelGGL((k_ilu_upper<3, TM, TV>), dim3(grid_for(nr, bs)), dim3(bs), 0, c->stream, (const TM *)im->LU,
-                           c->dm, im->color, im->crow + r0, nr, tmp, z);
+                           N, im->crow, im->cnb, im->clow, r0, nr, tmp, z);
     }
   }
   BES_HIP(hipGetLastError());
@@ -730,16 +905,16 @@ static int multidot(bes_ctx *c, Impl *im, const TV *V, size_t stride, int cnt, c
 template <typename TM, typename TV>
 static int gmres(bes_ctx *c, Impl *im, int *iters_out, double *relres_out) {
   const int m = im->restart;
-  const size_t n = (size_t)c->N * c->nv;
+  const size_t n = (size_t)c->N * c->nv, vs = im->vstride;
   TV *V = (TV *)im->V;
-  TV *Z = (TV *)im->Z, *Z1 = Z + n, *Z2 = Z + 2 * n;
+  TV *Z = (TV *)im->Z, *Z1 = Z + vs, *Z2 = Z + 2 * vs;
   hipMemsetAsync(im->x, 0, n * sizeof(double), c->stream);
   // ||b||
   double bnorm2 = 0;
   hipLaunchKernelGGL((k_scale<double, TV>), dim3(kRedBlocks), dim3(256), 0, c->stream, im->b, 1.0, Z, n);
   {
     double dummy;
-    int s = multidot<TV>(c, im, Z, n, 0, Z, n, &dummy, &bnorm2);
+    int s = multidot<TV>(c, im, Z, vs, 0, Z, n, &dummy, &bnorm2);
     if (s) return s;
   }
   const double bnorm = std::sqrt(bnorm2);
@@ -768,7 +943,7 @@ static int gmres(bes_ctx *c, Impl *im, int *iters_out, double *relres_out) {
       BES_HIP(hipStreamSynchronize(c->stream));
       rnorm = std::sqrt(r2);
       if (rnorm <= target) break;
-      hipLaunchKernelGGL((k_scale<TV, TV>), dim3(kRedBlocks), dim3(256), 0, c->stream, Z, 1.0 / rnorm, V, n);
+      hipLaunchKernelGGL((k_scale_p<TV>), dim3(kRedBlocks), dim3(256), 0, c->stream, Z, 1.0 / rnorm, V, n);
     }
     first = false;
     std::fill(g.begin(), g.end(), 0.0);
@@ -776,30 +951,29 @@ static int gmres(bes_ctx *c, Impl *im, int *iters_out, double *relres_out) {
     int j = 0;
     bool done = false;
     for (; j < m && total < im->maxiter; j++) {
-      TV *vj = V + (size_t)j * n, *w = V + (size_t)(j + 1) * n;
+      TV *vj = V + (size_t)j * vs, *w = V + (size_t)(j + 1) * vs;
       int s;
       if ((s = precond_apply<TM, TV>(c, im, vj, Z, Z1))) return s;
       if ((s = spmv<TM, TV>(c, im, Z, w))) return s;
       // classical Gram-Schmidt with one DGKS re-orthogonalisation
       double w0n2 = 0, wn2 = 0;
-      if ((s = multidot<TV>(c, im, V, n, j + 1, w, n, hcol.data(), &w0n2))) return s;
-      BES_HIP(hipMemcpyAsync(im->dots, hcol.data(), sizeof(double) * (j + 1), hipMemcpyHostToDevice, c->stream));
+      if ((s = multidot<TV>(c, im, V, vs, j + 1, w, n, hcol.data(), &w0n2))) return s;
       {
         ProfScope ps(c, kProfKrylov, (double)(j + 3) * n * sizeof(TV));
-        hipLaunchKernelGGL((k_orth_update<TV>), dim3(kRedBlocks), dim3(256), 0, c->stream, V, n, j + 1, im->dots + 0, w,
+        hipLaunchKernelGGL((k_orth_update<TV>), dim3(kRedBlocks), dim3(256), 0, c->stream, V, vs, j + 1, im->dots, w,
                            n, im->part);
         hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(256), 0, c->stream, im->part, kRedBlocks, 1, im->dots + 40);
       }
       BES_HIP(hipMemcpyAsync(&wn2, im->dots + 40, sizeof(double), hipMemcpyDeviceToHost, c->stream));
       BES_HIP(hipStreamSynchronize(c->stream));
-      if (std::sqrt(wn2) < 0.7 * std::sqrt(w0n2)) {
+      if (std::sqrt(wn2) < im->reorth_eta * std::sqrt(w0n2)) {
         std::vector<double> h2(j + 1);
-        if ((s = multidot<TV>(c, im, V, n, j + 1, w, n, h2.data(), nullptr))) return s;
+        if ((s = multidot<TV>(c, im, V, vs, j + 1, w, n, h2.data(), nullptr))) return s;
         for (int q = 0; q <= j; q++) hcol[q] += h2[q];
-        BES_HIP(hipMemcpyAsync(im->dots, h2.data(), sizeof(double) * (j + 1), hipMemcpyHostToDevice, c->stream));
+        im->reorths++;
         {
           ProfScope ps(c, kProfKrylov, (double)(j + 3) * n * sizeof(TV));
-          hipLaunchKernelGGL((k_orth_update<TV>), dim3(kRedBlocks), dim3(256), 0, c->stream, V, n, j + 1, im->dots + 0,
+          hipLaunchKernelGGL((k_orth_update<TV>), dim3(kRedBlocks), dim3(256), 0, c->stream, V, vs, j + 1, im->dots,
                              w, n, im->part);
           hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(256), 0, c->stream, im->part, kRedBlocks, 1, im->dots + 40);
         }
@@ -809,7 +983,7 @@ static int gmres(bes_ctx *c, Impl *im, int *iters_out, double *relres_out) {
       const double hn = std::sqrt(wn2);
       hcol[j + 1] = hn;
       if (hn > 0)
-        hipLaunchKernelGGL((k_scale<TV, TV>), dim3(kRedBlocks), dim3(256), 0, c->stream, w, 1.0 / hn, w, n);
+        hipLaunchKernelGGL((k_scale_p<TV>), dim3(kRedBlocks), dim3(256), 0, c->stream, w, 1.0 / hn, w, n);
       // Givens rotations
       for (int q = 0; q < j; q++) {
         const double t = cs[q] * hcol[q] + sn[q] * hcol[q + 1];
@@ -835,7 +1009,7 @@ static int gmres(bes_ctx *c, Impl *im, int *iters_out, double *relres_out) {
       y[q] = s / H[(size_t)q * m + q];
     }
     BES_HIP(hipMemcpyAsync(im->dots, y.data(), sizeof(double) * j, hipMemcpyHostToDevice, c->stream));
-    hipLaunchKernelGGL((k_combine<TV>), dim3(kRedBlocks), dim3(256), 0, c->stream, V, n, j, im->dots, Z2, n);
+    hipLaunchKernelGGL((k_combine<TV>), dim3(kRedBlocks), dim3(256), 0, c->stream, V, vs, j, im->dots, Z2, n);
     int s;
     if ((s = precond_apply<TM, TV>(c, im, Z2, Z, Z1))) return s;
     hipLaunchKernelGGL((k_axpy_x<TV>), dim3(kRedBlocks), dim3(256), 0, c->stream, im->x, Z, n);
@@ -892,6 +1066,19 @@ int bes_implicit_setup(bes_ctx *c, int precision, int precond, int restart, int 
   c->impl = im;
   int s = impl_alloc(c, im);
   if (s) { bes_implicit_free(c); return s; }
+  return BES_OK;
+}
+
+int bes_implicit_set_reorth(bes_ctx *c, double eta) {
+  if (!c || !c->impl || !(eta >= 0.0) || eta > 1.0) { set_error("call bes_implicit_setup first / eta in [0,1]"); return BES_ERR_ARG; }
+  impl_of(c)->reorth_eta = eta;
+  return BES_OK;
+}
+
+int bes_implicit_stats(bes_ctx *c, long *reorths, int *lp_fallbacks) {
+  if (!c || !c->impl) { set_error("call bes_implicit_setup first"); return BES_ERR_ARG; }
+  if (reorths) *reorths = impl_of(c)->reorths;
+  if (lp_fallbacks) *lp_fallbacks = impl_of(c)->lp_fallbacks;
   return BES_OK;
 }
 
@@ -1098,43 +1285,70 @@ int bes_implicit_update(bes_ctx *c) {
 // SolverImplicit::solve loop (SolverImplicit.cpp:552-600) with the adaptive
 // Courant number of SolverImplicit.cpp:530-550.  hist [nsteps][2*nv+3]:
 // Linf(b) per var, Linf(sum F*A) per var, CFL used, Krylov iterations, relres.
-int bes_implicit_run(bes_ctx *c, int nsteps, double cfl_min, double cfl_max, const double *exponents, int local,
-                     int repeat_nv, double stop_drop, double *hist, int *steps_done) {
-  if (!c || !c->impl || nsteps < 0 || !exponents) { set_error("invalid argument"); return BES_ERR_ARG; }
+// nsteps more pseudo-time steps of the loop in Impl::run
+static int run_steps(bes_ctx *c, Impl *im, int nsteps, double *hist, int *steps_done) {
   const int nv = c->nv;
-  double cfl = cfl_min;
-  std::vector<double> lb(nv), lr(nv), lb_old(nv, 0.0);
-  double r0 = -1;
+  Impl::Run &R = im->run;
+  std::vector<double> lb(nv), lr(nv);
   int done = 0;
-  for (int it = 0; it < nsteps; it++) {
-    int s = bes_implicit_assemble(c, cfl, local, repeat_nv, lb.data(), lr.data());
-    if (s) return s;
+  for (int it = 0; it < nsteps && !R.reached; it++) {
+    int s = bes_implicit_assemble(c, R.cfl, R.local, R.repeat_nv, lb.data(), lr.data());
+    if (s) { if (steps_done) *steps_done = done; return s; }
     int iters = 0;
     double rr = 0;
     s = bes_linear_solve(c, &iters, &rr);
-    if (s && s != BES_ERR_SOLVE) return s;
-    if ((s = bes_implicit_update(c))) return s;
+    if (s && s != BES_ERR_SOLVE) { if (steps_done) *steps_done = done; return s; }
+    if ((s = bes_implicit_update(c))) { if (steps_done) *steps_done = done; return s; }
     if (hist) {
       double *h = hist + (size_t)it * (2 * nv + 3);
       for (int v = 0; v < nv; v++) { h[v] = lb[v]; h[nv + v] = lr[v]; }
-      h[2 * nv] = cfl;
+      h[2 * nv] = R.cfl;
       h[2 * nv + 1] = iters;
       h[2 * nv + 2] = rr;
     }
     done = it + 1;
-    if (r0 < 0) r0 = lr[0];
+    if (R.r0 < 0) R.r0 = lr[0];
     // adaptive Courant number (density monitor of the implicit residual)
-    if (cfl_max > cfl_min) {
-      const int ei = (lb_old[0] > lb[0]) ? 1 : 0;
-      cfl *= std::pow(lb_old[0] / lb[0], exponents[ei]);
-      cfl = std::max(cfl_min, cfl);
-      cfl = std::min(cfl_max, cfl);
+    if (R.cfl_max > R.cfl_min) {
+      const int ei = (R.lb_old[0] > lb[0]) ? 1 : 0;
+      R.cfl *= std::pow(R.lb_old[0] / lb[0], R.ex[ei]);
+      R.cfl = std::max(R.cfl_min, R.cfl);
+      R.cfl = std::min(R.cfl_max, R.cfl);
     }
-    lb_old = lb;
-    if (stop_drop > 0 && r0 > 0 && lr[0] <= stop_drop * r0) break;
+    R.lb_old = lb;
+    if (R.stop_drop > 0 && R.r0 > 0 && lr[0] <= R.stop_drop * R.r0) R.reached = true;
   }
   if (steps_done) *steps_done = done;
   return BES_OK;
+}
+
+int bes_implicit_run(bes_ctx *c, int nsteps, double cfl_min, double cfl_max, const double *exponents, int local,
+                     int repeat_nv, double stop_drop, double *hist, int *steps_done) {
+  if (!c || !c->impl || nsteps < 0 || !exponents) { set_error("invalid argument"); return BES_ERR_ARG; }
+  Impl *im = impl_of(c);
+  Impl::Run &R = im->run;
+  R = Impl::Run();
+  R.active = true;
+  R.cfl = R.cfl_min = cfl_min;
+  R.cfl_max = cfl_max;
+  R.ex[0] = exponents[0];
+  R.ex[1] = exponents[1];
+  R.local = local;
+  R.repeat_nv = repeat_nv;
+  R.stop_drop = stop_drop;
+  R.lb_old.assign(c->nv, 0.0);
+  return run_steps(c, im, nsteps, hist, steps_done);
+}
+
+int bes_implicit_continue(bes_ctx *c, int nsteps, double *hist, int *steps_done, int *reached) {
+  if (!c || !c->impl || nsteps < 0 || !impl_of(c)->run.active) {
+    set_error("no pseudo-time loop to continue (bes_implicit_run first)");
+    return BES_ERR_ARG;
+  }
+  Impl *im = impl_of(c);
+  int s = run_steps(c, im, nsteps, hist, steps_done);
+  if (reached) *reached = im->run.reached ? 1 : 0;
+  return s;
 }
 
 }  // extern "C"

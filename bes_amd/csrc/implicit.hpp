@@ -43,19 +43,34 @@ struct Impl {
   void *LM = nullptr;      // learned operator blocks (TM), [(k+1)*nv*nv][N] (learned.hip)
   void *lp = nullptr;      // learned-preconditioner network state (learned.hip)
   int lp_fallbacks = 0;    // solves that fell back to ILU(0)
-  void *V = nullptr;       // Krylov basis (TV) [(m+1)][N*nv]
-  void *Z = nullptr;       // work vectors (TV) [3][N*nv]
+  void *V = nullptr;       // Krylov basis (TV) [(m+1)][vstride]
+  void *Z = nullptr;       // work vectors (TV) [3][vstride]
+  size_t vstride = 0;      // N*nv rounded up to 64 elements
+  double reorth_eta = 0.7; // DGKS re-orthogonalisation when ||w|| drops below eta of its norm
+  long reorths = 0;        // re-orthogonalisations so far
   double *part = nullptr;  // reduction partials [32][kRedBlocks]
   double *dots = nullptr;  // [64]
   unsigned long long *bits = nullptr;   // [16] Linf monitors
   int ncolor = 0;
   std::vector<int> color_start;   // host: rows of colour c are crow[color_start[c] .. color_start[c+1])
-  int *crow = nullptr;            // [N] rows grouped by colour
+  int *crow = nullptr;            // [N] rows grouped by colour: crow[p] = cell at position p
   int *color = nullptr;           // [N]
+  // ILU(0) factors live in colour-major order: LU[slot][e][p]; per position p
+  int *cpos = nullptr;            // [N] position of cell i
+  int *cnb = nullptr;             // [k][N] neighbour cell through local face m of the cell at p (-1 none)
+  int *cnp = nullptr;             // [k][N] that neighbour's position
+  unsigned char *clow = nullptr;  // [N] bit m: neighbour m precedes in the elimination order
   long ninternal = 0;              // internal faces (off-diagonal block pairs)
   int last_iters = 0;
   double last_relres = 0;
   bool assembled = false, factored = false;
+  // pseudo-time loop state (bes_implicit_run / bes_implicit_continue)
+  struct Run {
+    bool active = false, reached = false;
+    double cfl = 0, cfl_min = 0, cfl_max = 0, ex[2] = {1, 1}, stop_drop = 0, r0 = -1;
+    int local = 1, repeat_nv = 1;
+    std::vector<double> lb_old;
+  } run;
 };
 
 // ---------------------------------------------------------------------------

@@ -331,8 +331,12 @@ def swept_wing(ni=96, nj=48, nk=72, seed=20261202, spec: WingSpec = None, jitter
     columns between the surface and the far field are compressed so every
     hexahedron stays valid; cells cluster geometrically toward the wing.
     Interior nodes away from the wing get a seeded jitter.  Boundaries: the
-    root plane y = 0 (symmetry) and the wing are slip walls, the outer box is
-    freestream.  x is chordwise, y spanwise, z up.
+    root plane y = 0, the outboard side plane and the wing are slip walls; the
+    upstream, downstream, bottom and top planes are freestream.  The first
+    vertical spacing grows with the in-plane distance from the planform and
+    the chordwise clustering relaxes away from the wing plane and outboard of
+    the tip, so far-field cells stay moderately shaped.  x is chordwise, y
+    spanwise, z up.
     """
     spec = spec or WingSpec()
     rng = np.random.default_rng(seed)
@@ -361,15 +365,17 @@ def swept_wing(ni=96, nj=48, nk=72, seed=20261202, spec: WingSpec = None, jitter
 
     # chordwise parameter on the wing (cosine clustering at LE and TE)
     xi = 0.5 * (1.0 - np.cos(np.pi * np.linspace(0.0, 1.0, nchord + 1)))
-    # vertical distribution from the slit plane
+    # vertical distribution from the slit plane: first spacing hz0 over the
+    # planform, growing with the in-plane distance d from it, so cells far
+    # from the wing are not flattened by the clustering near z = 0
     hz0 = 0.004 * c_r
-    z_up = _stretch(nk - kS, hz0, ff)
-    z_dn = _stretch(kS, hz0, ff)
-    Zt = np.concatenate([-z_dn[::-1], [0.0], z_up])   # nk+1 values, Zt[kS] = 0
+    hz_cap = 0.5 * ff / max(kS, 1)
 
     NI, NJ, NK = ni + 1, nj + 1, nk + 1
-    X = np.zeros((NI, NJ))
+    X = np.zeros((NI, NJ))       # chordwise positions on the wing plane (clustered)
+    Xs = np.zeros((NI, NJ))      # the same without LE/TE clustering, used away from it
     Zs_up = np.zeros((NI, NJ))   # surface z of the upper copy (0 off the wing)
+    xu = np.linspace(0.0, 1.0, nchord + 1)
     for j in range(NJ):
         y = Y[j]
         xl, c = x_le(y), chord(y)
@@ -379,6 +385,13 @@ def swept_wing(ni=96, nj=48, nk=72, seed=20261202, spec: WingSpec = None, jitter
         x_up = xl - _stretch(iLE, h_le, ff + xl)[::-1]
         x_dn = xl + c + _stretch(ni - iTE, h_te, ff + c_r - (xl + c))
         X[:, j] = np.concatenate([x_up, xw, x_dn])
+        hu = c / nchord
+        Xs[:, j] = np.concatenate([xl - _stretch(iLE, hu, ff + xl)[::-1], xl + c * xu,
+                                   xl + c + _stretch(ni - iTE, hu, ff + c_r - (xl + c))])
+        # outboard of the tip the LE/TE clustering relaxes over half a root chord
+        g = np.clip((y - s_half) / (0.5 * c_r), 0.0, 1.0)
+        g = g * g * (3.0 - 2.0 * g)
+        X[:, j] = (1.0 - g) * X[:, j] + g * Xs[:, j]
         if j <= jt:
             yc = spec.tip_cap * s_half
             tau = 1.0 if y <= s_half - yc else np.sqrt(max(0.0, 1.0 - ((y - (s_half - yc)) / yc) ** 2))
@@ -388,13 +401,37 @@ def swept_wing(ni=96, nj=48, nk=72, seed=20261202, spec: WingSpec = None, jitter
     Zs_up[iLE, :] = 0.0
     Zs_up[iTE, :] = 0.0
 
+    # per-column z levels (nk+1 values, 0 at k = kS)
+    Zt = np.zeros((NI, NJ, NK))
+    cache = {}
+    for j in range(NJ):
+        y = Y[j]
+        xl, c = x_le(y), chord(y)
+        dy = max(y - s_half, 0.0)
+        for i in range(NI):
+            x = X[i, j]
+            dx = max(xl - x, 0.0, x - (xl + c))
+            d = np.hypot(dx, dy)
+            h0 = min(hz0 * (1.0 + d / (0.05 * c_r)), hz_cap)
+            key = round(h0, 12)
+            if key not in cache:
+                z_up = _stretch(nk - kS, h0, ff)
+                z_dn = _stretch(kS, h0, ff)
+                cache[key] = np.concatenate([-z_dn[::-1], [0.0], z_up])
+            Zt[i, j] = cache[key]
+
     # node coordinates of the full lattice (upper copies on the slit plane)
     P = np.zeros((NI, NJ, NK, 3))
-    P[..., 0] = X[:, :, None]
+    # blend from the clustered to the unclustered chordwise distribution over
+    # kS/3 layers, starting 3 layers off the wing plane (same weight along a lattice
+    # row, so x stays monotone in i)
+    tk = np.clip((np.abs(np.arange(NK) - kS) - 3) / max(kS / 3.0, 1.0), 0.0, 1.0)
+    beta = tk * tk * (3.0 - 2.0 * tk)
+    P[..., 0] = (1.0 - beta)[None, None, :] * X[:, :, None] + beta[None, None, :] * Xs[:, :, None]
     P[..., 1] = Y[None, :, None]
     zs = Zs_up[:, :, None]
-    ztop, zbot = Zt[-1], -Zt[0]
-    zk = Zt[None, None, :]
+    ztop, zbot = ff, ff
+    zk = Zt
     upper = zk >= 0
     P[..., 2] = np.where(upper, zs + zk * (1.0 - zs / ztop), -zs + zk * (1.0 - zs / zbot))
     # seeded jitter of interior nodes away from the wing and the boundaries,
@@ -468,9 +505,12 @@ def swept_wing(ni=96, nj=48, nk=72, seed=20261202, spec: WingSpec = None, jitter
     bf = boundary_faces_of(tets, 3)
     cen = nodes[bf].mean(axis=1)
     eps = 1e-9
+    # far field: upstream, downstream, bottom and top planes.  The outboard
+    # side plane runs parallel to the freestream, where the characteristic
+    # condition would switch between its inflow and outflow branches on the
+    # sign of a vanishing normal velocity; it is a symmetry (slip) plane.
     outer = ((np.abs(cen[:, 0] - nodes[:, 0].min()) < eps) | (np.abs(cen[:, 0] - nodes[:, 0].max()) < eps) |
-             (np.abs(cen[:, 1] - Y[-1]) < eps) | (np.abs(cen[:, 2] - Zt[0]) < eps) |
-             (np.abs(cen[:, 2] - Zt[-1]) < eps))
+             (np.abs(cen[:, 2] + ff) < eps) | (np.abs(cen[:, 2] - ff) < eps))
     btype = np.where(outer, FREESTREAM, SLIP_WALL).astype(np.int32)
     md = MeshData(3, nodes, tets.astype(np.int64), bf.astype(np.int64), btype)
     md.wing = dict(spec=spec, iLE=iLE, iTE=iTE, jt=jt, kS=kS)

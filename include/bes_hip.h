@@ -154,6 +154,11 @@ int bes_jacobian_batch(int ndim, int kind, long n, const double *UL, const doubl
 int bes_implicit_setup(bes_ctx *ctx, int precision, int precond, int restart, int maxiter, double rtol);
 /* switch the preconditioner (0-3) of an existing setup; the next solve factors again */
 int bes_implicit_set_precond(bes_ctx *ctx, int precond);
+/* classical Gram-Schmidt re-orthogonalises (DGKS) when the new Krylov vector
+ * keeps less than eta of its norm (default 0.7; 0 never) */
+int bes_implicit_set_reorth(bes_ctx *ctx, double eta);
+/* counters since setup: re-orthogonalisations, learned solves that fell back */
+int bes_implicit_stats(bes_ctx *ctx, long *reorths, int *lp_fallbacks);
 /* time step, gradient, limiter, face Jacobians and the system A x = b at the
  * current state (SolverImplicit.cpp:307-320 = resetSystemOfEquation_ + dt +
  * gradient + limiter + computeFaceFluxes_).  repeat_nv != 0 reproduces the
@@ -185,6 +190,10 @@ int bes_implicit_update(bes_ctx *ctx);
  * var, Linf(sum F*A) per var, Courant number, Krylov iterations, relres. */
 int bes_implicit_run(bes_ctx *ctx, int nsteps, double cfl_min, double cfl_max, const double *exponents,
                      int local, int repeat_nv, double stop_drop, double *hist, int *steps_done);
+/* nsteps more steps of the loop the last bes_implicit_run started (same
+ * parameters; Courant number and residual history carry over), stopping early
+ * at its stop_drop; reached = 1 once the drop has been reached. */
+int bes_implicit_continue(bes_ctx *ctx, int nsteps, double *hist, int *steps_done, int *reached);
 
 /* ---- learned preconditioner (no reference counterpart; an alternative to the
  *      AMGCL preconditioner of CoefficientMatrix.cpp:210-325, used with
