@@ -58,9 +58,13 @@ def roofline(prof):
     dom = max(kern, key=lambda k: kern[k]["ms"])
     d = kern[dom]
     avg_ms = d["ms"] / d["launches"]
-    return {"bound": "hbm", "kernel": KERNEL_NAMES.get(dom, dom), "achieved": round(d["GBps"], 1),
+    name = KERNEL_NAMES.get(dom, dom)
+    tr = traffic_from_profiles(name)
+    return {"bound": "hbm", "kernel": name, "achieved": round(d["GBps"], 1),
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(d["GBps"] / HBM_PEAK_GBS, 4),
-            "traffic": None, "alg_bytes_per_launch": round(d["bytes"] / d["launches"]),
+            "traffic": tr["bytes_per_launch"] if tr else None,
+            "traffic_source": tr["source"] if tr else None,
+            "alg_bytes_per_launch": round(d["bytes"] / d["launches"]),
             "avg_launch_us": round(avg_ms * 1e3, 2),
             "kernels": {KERNEL_NAMES.get(k, k): {"total_ms": round(v["ms"], 3), "launches": v["launches"],
                                                   "avg_us": round(v["ms"] / v["launches"] * 1e3, 2),
@@ -379,6 +383,29 @@ def cpu_baseline_explicit(args, nd):
                       f"{dt:.1f} s, oracle OpenMP face loop with atomics as the reference"}
 
 
+def copy_probe():
+    """A plain device copy of known size: 1 GiB read and 1 GiB written per copy."""
+    import torch
+    n = 1 << 28
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": n * 4, "copies": 5}))
+
+
+def traffic_from_profiles(kernel):
+    """HBM bytes per launch of a kernel as measured by the committed counter runs."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        table = json.load(fh)
+    return table.get(kernel)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -398,11 +425,16 @@ def main():
                     help="exact first-order Jacobian (the reference adds each block nV times)")
     ap.add_argument("--repeat", dest="no_repeat", action="store_false")
     ap.add_argument("--quick-drop", action="store_true", help="skip the mixed FP32 ILU(0) drop run")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only copy 1 GiB device to device 5 times (calibrates HBM byte counters)")
     ap.add_argument("--rtol", type=float, default=1e-3)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-drop", action="store_true")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe()
+        return
     implicit = args.workload.startswith("implicit")
     if args.steps is None:
         args.steps = 30 if implicit else 20
