@@ -188,7 +188,8 @@ class Config {
     // keys of this build
     else if (k == "GEOMETRY_MODE") geomMode_ = a[0] == "EXACT" ? BES_GEOM_EXACT : BES_GEOM_COMPAT;
     else if (k == "LINEAR_SOLVER_PRECISION") precision_ = (int)num(a[0]);
-    else if (k == "PRECONDITIONER") precond_ = a[0] == "NONE" ? 0 : (a[0] == "BLOCK_JACOBI" ? 1 : 2);
+    else if (k == "PRECONDITIONER")
+      precond_ = a[0] == "NONE" ? 0 : a[0] == "BLOCK_JACOBI" ? 1 : a[0] == "LEARNED" ? 3 : 2;
     else if (k == "JACOBIAN_BLOCK_REPEAT") blockRepeat_ = a[0] == "NO" ? 0 : 1;
     else throw Error(BES_ERR_ARG, "unknown config key " + k);
   }
@@ -451,6 +452,7 @@ class CoefficientMatrix {
     nv_ = config->getNumConservedVariable();
     check(bes_implicit_setup(ctx_, config->getPrecision(), config->getPreconditioner(), 30,
                              config->getLinearSolverNumIter(), config->getLinearSolverTol()));
+    if (config->getPreconditioner() == 3) check(bes_lp_init(ctx_, 20261202ull));
     const long nnz = bes_implicit_nnz(ctx_);
     const long nrow = (long)mesh->getNumElement() * nv_;
     row_.assign(nrow + 1, 0);
@@ -581,6 +583,7 @@ class SolverImplicit : public Solver {
     initializeBase_(config, gas, mesh);
     check(bes_implicit_setup(mesh->ctx(), config->getPrecision(), config->getPreconditioner(), 30,
                              config->getLinearSolverNumIter(), config->getLinearSolverTol()));
+    if (config->getPreconditioner() == 3) check(bes_lp_init(mesh->ctx(), 20261202ull));
   }
   void setUniversalInitialCondition(double *primitive) override { uniformState_(primitive); }
   // SolverImplicit.cpp:552-600 with the adaptive CFL of :530-550
