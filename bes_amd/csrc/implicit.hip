@@ -188,14 +188,65 @@ __global__ __launch_bounds__(256) void k_bj_setup(const TM *__restrict__ A, TM *
 // block ILU(0) factorisation of the rows of one colour (all lower colours done).
 // LU holds a copy of A on entry; on exit slot 0 holds inv(U_ii), lower slots L_ik,
 // upper slots U_ij.
-// colour-major copy of the matrix: LU[slot][e][p] = A[slot][e][crow[p]]
+// colour-major copies of the matrix: Ac[slot][e][p] = LU[slot][e][p] = A[slot][e][crow[p]]
 template <typename TM>
 __global__ void k_ilu_gather(const TM *__restrict__ A, const int *__restrict__ crow, int N, int nslot_e,
-                             TM *__restrict__ LU) {
+                             TM *__restrict__ Ac, TM *__restrict__ LU) {
   const size_t tot = (size_t)nslot_e * N;
   for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < tot; q += (size_t)gridDim.x * blockDim.x) {
     const size_t se = q / N, p = q - se * N;
-    LU[q] = A[se * N + crow[p]];
+    const TM v = A[se * N + crow[p]];
+    Ac[q] = v;
+    LU[q] = v;
+  }
+}
+
+// y = A x with matrix and vectors in colour order (neighbour positions cnp)
+template <int ND, typename TM, typename TV>
+__global__ __launch_bounds__(256) void k_spmv_pos(const TM *__restrict__ A, const int *__restrict__ cnp, int N,
+                                                  const TV *__restrict__ x, TV *__restrict__ y) {
+  constexpr int NV = ND + 2, K = ND + 1, NB = NV * NV;
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= N) return;
+  double acc[NV], xv[NV];
+#pragma unroll
+  for (int c = 0; c < NV; c++) xv[c] = (double)x[(size_t)p * NV + c];
+#pragma unroll
+  for (int r = 0; r < NV; r++) {
+    double s = 0;
+#pragma unroll
+    for (int c = 0; c < NV; c++) s += (double)A[((size_t)(r * NV + c)) * N + p] * xv[c];
+    acc[r] = s;
+  }
+#pragma unroll
+  for (int m = 0; m < K; m++) {
+    const int q = cnp[(size_t)m * N + p];
+    if (q < 0) continue;
+#pragma unroll
+    for (int c = 0; c < NV; c++) xv[c] = (double)x[(size_t)q * NV + c];
+    const TM *Ao = A + (size_t)(1 + m) * NB * N;
+#pragma unroll
+    for (int r = 0; r < NV; r++) {
+      double s = 0;
+#pragma unroll
+      for (int c = 0; c < NV; c++) s += (double)Ao[((size_t)(r * NV + c)) * N + p] * xv[c];
+      acc[r] += s;
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < NV; r++) y[(size_t)p * NV + r] = (TV)acc[r];
+}
+
+// rows between cell order and colour order: to_colour: dst[p] = src[crow[p]], else dst[crow[p]] = src[p]
+template <typename T>
+__global__ void k_perm_rows(const T *__restrict__ src, const int *__restrict__ crow, int N, int nv, int to_colour,
+                            T *__restrict__ dst) {
+  const size_t tot = (size_t)N * nv;
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < tot; q += (size_t)gridDim.x * blockDim.x) {
+    const size_t p = q / nv, c = q - p * nv;
+    const size_t i = (size_t)crow[p];
+    if (to_colour) dst[q] = src[i * nv + c];
+    else dst[i * nv + c] = src[q];
   }
 }
 
@@ -288,27 +339,26 @@ __global__ __launch_bounds__(256) void k_ilu_factor_color(TM *__restrict__ LU, i
   store_block<TM, NB>(LU, 0, N, p, Inv);
 }
 
-// forward sweep of one colour: y_i = r_i - sum_(k lower) L_ik y_k
+// forward sweep of one colour (vectors in colour order): y_p = r_p - sum_(k lower) L_pk y_k
 template <int ND, typename TM, typename TV>
-__global__ __launch_bounds__(256) void k_ilu_lower(const TM *__restrict__ LU, int N, const int *__restrict__ crow,
-                                                   const int *__restrict__ cnb, const unsigned char *__restrict__ clow,
-                                                   int p0, int nrows, const TV *__restrict__ r, TV *__restrict__ y) {
+__global__ __launch_bounds__(256) void k_ilu_lower(const TM *__restrict__ LU, int N, const int *__restrict__ cnp,
+                                                   const unsigned char *__restrict__ clow, int p0, int nrows,
+                                                   const TV *__restrict__ r, TV *__restrict__ y) {
   constexpr int NV = ND + 2, K = ND + 1, NB = NV * NV;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nrows) return;
   const int p = p0 + t;
-  const int i = crow[p];
   const unsigned lowm = clow[p];
   double acc[NV];
 #pragma unroll
-  for (int c = 0; c < NV; c++) acc[c] = (double)r[(size_t)i * NV + c];
+  for (int c = 0; c < NV; c++) acc[c] = (double)r[(size_t)p * NV + c];
 #pragma unroll
   for (int m = 0; m < K; m++) {
     if (!((lowm >> m) & 1)) continue;
-    const int k = cnb[(size_t)m * N + p];
+    const int q = cnp[(size_t)m * N + p];
     double yk[NV];
 #pragma unroll
-    for (int c = 0; c < NV; c++) yk[c] = (double)y[(size_t)k * NV + c];
+    for (int c = 0; c < NV; c++) yk[c] = (double)y[(size_t)q * NV + c];
     const TM *L = LU + (size_t)(1 + m) * NB * N;
 #pragma unroll
     for (int a = 0; a < NV; a++) {
@@ -319,30 +369,29 @@ __global__ __launch_bounds__(256) void k_ilu_lower(const TM *__restrict__ LU, in
     }
   }
 #pragma unroll
-  for (int c = 0; c < NV; c++) y[(size_t)i * NV + c] = (TV)acc[c];
+  for (int c = 0; c < NV; c++) y[(size_t)p * NV + c] = (TV)acc[c];
 }
 
-// backward sweep of one colour: x_i = inv(U_ii) (y_i - sum_(j upper) U_ij x_j)
+// backward sweep of one colour (vectors in colour order): x_p = inv(U_pp) (y_p - sum_(j upper) U_pj x_j)
 template <int ND, typename TM, typename TV>
-__global__ __launch_bounds__(256) void k_ilu_upper(const TM *__restrict__ LU, int N, const int *__restrict__ crow,
-                                                   const int *__restrict__ cnb, const unsigned char *__restrict__ clow,
-                                                   int p0, int nrows, const TV *__restrict__ y, TV *__restrict__ x) {
+__global__ __launch_bounds__(256) void k_ilu_upper(const TM *__restrict__ LU, int N, const int *__restrict__ cnp,
+                                                   const unsigned char *__restrict__ clow, int p0, int nrows,
+                                                   const TV *__restrict__ y, TV *__restrict__ x) {
   constexpr int NV = ND + 2, K = ND + 1, NB = NV * NV;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nrows) return;
   const int p = p0 + t;
-  const int i = crow[p];
   const unsigned lowm = clow[p];
   double acc[NV];
 #pragma unroll
-  for (int c = 0; c < NV; c++) acc[c] = (double)y[(size_t)i * NV + c];
+  for (int c = 0; c < NV; c++) acc[c] = (double)y[(size_t)p * NV + c];
 #pragma unroll
   for (int m = 0; m < K; m++) {
-    const int j = cnb[(size_t)m * N + p];
-    if (j < 0 || ((lowm >> m) & 1)) continue;
+    const int q = cnp[(size_t)m * N + p];
+    if (q < 0 || ((lowm >> m) & 1)) continue;
     double xj[NV];
 #pragma unroll
-    for (int c = 0; c < NV; c++) xj[c] = (double)x[(size_t)j * NV + c];
+    for (int c = 0; c < NV; c++) xj[c] = (double)x[(size_t)q * NV + c];
     const TM *Uo = LU + (size_t)(1 + m) * NB * N;
 #pragma unroll
     for (int a = 0; a < NV; a++) {
@@ -357,7 +406,7 @@ __global__ __launch_bounds__(256) void k_ilu_upper(const TM *__restrict__ LU, in
     double s = 0;
 #pragma unroll
     for (int c = 0; c < NV; c++) s += (double)LU[((size_t)(a * NV + c)) * N + p] * acc[c];
-    x[(size_t)i * NV + a] = (TV)s;
+    x[(size_t)p * NV + a] = (TV)s;
   }
 }
 
@@ -733,6 +782,9 @@ static int impl_alloc(bes_ctx *c, Impl *im) {
   BES_HIP(hipMalloc((void **)&im->x, nvec * sizeof(double)));
   BES_HIP(hipMalloc(&im->A, mat * es));
   BES_HIP(hipMalloc(&im->LU, mat * es));
+  BES_HIP(hipMalloc(&im->Ac, mat * es));
+  BES_HIP(hipMalloc((void **)&im->bc, nvec * sizeof(double)));
+  BES_HIP(hipMalloc((void **)&im->xc, nvec * sizeof(double)));
   im->vstride = (nvec + 63) / 64 * 64;   // 16-byte aligned packet loads on every vector
   BES_HIP(hipMalloc(&im->V, (size_t)(im->restart + 1) * im->vstride * es));
   BES_HIP(hipMalloc(&im->Z, 3 * im->vstride * es));
@@ -745,7 +797,7 @@ static int impl_alloc(bes_ctx *c, Impl *im) {
 static void impl_release(Impl *im) {
   lp::lp_release(im);
   void *ptrs[] = {im->fd.fo, im->fd.fn, im->fd.fnrm, im->fd.cf, im->fd.mrev, im->color, im->crow, im->JF,
-                  im->b, im->x, im->A, im->LU, im->LM, im->V, im->Z, im->part, im->dots, im->bits,
+                  im->b, im->x, im->A, im->LU, im->LM, im->V, im->Z, im->part, im->dots, im->bits, im->Ac, im->bc, im->xc,
                   im->cpos, im->cnb, im->cnp, im->clow};
   for (void *p : ptrs)
     if (p) hipFree(p);
@@ -803,7 +855,7 @@ static int factor(bes_ctx *c, Impl *im) {
     else hipLaunchKernelGGL((k_bj_setup<3, TM>), dim3(grid_for(N, bs)), dim3(bs), 0, c->stream, (const TM *)im->A, (TM *)im->LU, N);
   } else if (im->precond == 2) {
     hipLaunchKernelGGL((k_ilu_gather<TM>), dim3(2048), dim3(256), 0, c->stream, (const TM *)im->A, im->crow, N,
-                       (c->k + 1) * nv * nv, (TM *)im->LU);
+                       (c->k + 1) * nv * nv, (TM *)im->Ac, (TM *)im->LU);
     for (int col = 0; col < im->ncolor; col++) {
       const int r0 = im->color_start[col], nr = im->color_start[col + 1] - r0;
       if (nr == 0) continue;
@@ -845,20 +897,20 @@ static int precond_apply(bes_ctx *c, Impl *im, const TV *r, TV *z, TV *tmp) {
       if (nr == 0) continue;
       if (c->nd == 2)
         hipLaunchKernelGGL((k_ilu_lower<2, TM, TV>), dim3(grid_for(nr, bs)), dim3(bs), 0, c->stream, (const TM *)im->LU,
-                           N, im->crow, im->cnb, im->clow, r0, nr, r, tmp);
+                           N, im->cnp, im->clow, r0, nr, r, tmp);
       else
         hipLaunchKernelGGL((k_ilu_lower<3, TM, TV>), dim3(grid_for(nr, bs)), dim3(bs), 0, c->stream, (const TM *)im->LU,
-                           N, im->crow, im->cnb, im->clow, r0, nr, r, tmp);
+                           N, im->cnp, im->clow, r0, nr, r, tmp);
     }
     for (int col = im->ncolor - 1; col >= 0; col--) {
       const int r0 = im->color_start[col], nr = im->color_start[col + 1] - r0;
       if (nr == 0) continue;
       if (c->nd == 2)
         hipLaunchKernelGGL((k_ilu_upper<2, TM, TV>), dim3(grid_for(nr, bs)), dim3(bs), 0, c->stream, (const TM *)im->LU,
-                           N, im->crow, im->cnb, im->clow, r0, nr, tmp, z);
+                           N, im->cnp, im->clow, r0, nr, tmp, z);
       else
         hipLaunchKernelGGL((k_ilu_upper<3, TM, TV>), dim3(grid_for(nr, bs)), dim3(bs), 0, c->stream, (const TM *)im->LU,
-                           N, im->crow, im->cnb, im->clow, r0, nr, tmp, z);
+                           N, im->cnp, im->clow, r0, nr, tmp, z);
     }
   }
   BES_HIP(hipGetLastError());
@@ -871,8 +923,18 @@ static int spmv(bes_ctx *c, Impl *im, const TV *x, TV *y) {
   // stored blocks, x and y once, neighbour ids
   ProfScope ps(c, kProfSpmv, (double)(c->N + 2 * im->ninternal) * c->nv * c->nv * sizeof(TM) +
                                  2.0 * c->N * c->nv * sizeof(TV) + 4.0 * c->N * c->k);
-  if (c->nd == 2) hipLaunchKernelGGL((k_spmv<2, TM, TV>), dim3(grid_for(c->N, bs)), dim3(bs), 0, c->stream, (const TM *)im->A, c->dm, x, y);
-  else hipLaunchKernelGGL((k_spmv<3, TM, TV>), dim3(grid_for(c->N, bs)), dim3(bs), 0, c->stream, (const TM *)im->A, c->dm, x, y);
+  if (im->precond == 2) {   // ILU(0) solves run in colour order
+    if (c->nd == 2)
+      hipLaunchKernelGGL((k_spmv_pos<2, TM, TV>), dim3(grid_for(c->N, bs)), dim3(bs), 0, c->stream, (const TM *)im->Ac,
+                         im->cnp, c->N, x, y);
+    else
+      hipLaunchKernelGGL((k_spmv_pos<3, TM, TV>), dim3(grid_for(c->N, bs)), dim3(bs), 0, c->stream, (const TM *)im->Ac,
+                         im->cnp, c->N, x, y);
+  } else if (c->nd == 2) {
+    hipLaunchKernelGGL((k_spmv<2, TM, TV>), dim3(grid_for(c->N, bs)), dim3(bs), 0, c->stream, (const TM *)im->A, c->dm, x, y);
+  } else {
+    hipLaunchKernelGGL((k_spmv<3, TM, TV>), dim3(grid_for(c->N, bs)), dim3(bs), 0, c->stream, (const TM *)im->A, c->dm, x, y);
+  }
   BES_HIP(hipGetLastError());
   return BES_OK;
 }
@@ -908,10 +970,19 @@ static int gmres(bes_ctx *c, Impl *im, int *iters_out, double *relres_out) {
   const size_t n = (size_t)c->N * c->nv, vs = im->vstride;
   TV *V = (TV *)im->V;
   TV *Z = (TV *)im->Z, *Z1 = Z + vs, *Z2 = Z + 2 * vs;
-  hipMemsetAsync(im->x, 0, n * sizeof(double), c->stream);
+  // ILU(0) solves run in colour order: b in, x out are permuted once per solve
+  const bool colour = im->precond == 2;
+  const double *bb = im->b;
+  double *xx = colour ? im->xc : im->x;
+  if (colour) {
+    hipLaunchKernelGGL(k_perm_rows<double>, dim3(1024), dim3(256), 0, c->stream, bb, im->crow, c->N, c->nv, 1,
+                       im->bc);
+    bb = im->bc;
+  }
+  hipMemsetAsync(xx, 0, n * sizeof(double), c->stream);
   // ||b||
   double bnorm2 = 0;
-  hipLaunchKernelGGL((k_scale<double, TV>), dim3(kRedBlocks), dim3(256), 0, c->stream, im->b, 1.0, Z, n);
+  hipLaunchKernelGGL((k_scale<double, TV>), dim3(kRedBlocks), dim3(256), 0, c->stream, bb, 1.0, Z, n);
   {
     double dummy;
     int s = multidot<TV>(c, im, Z, vs, 0, Z, n, &dummy, &bnorm2);
@@ -920,7 +991,10 @@ static int gmres(bes_ctx *c, Impl *im, int *iters_out, double *relres_out) {
   const double bnorm = std::sqrt(bnorm2);
   *iters_out = 0;
   *relres_out = 0;
-  if (bnorm == 0) return BES_OK;
+  if (bnorm == 0) {
+    if (colour) hipMemsetAsync(im->x, 0, n * sizeof(double), c->stream);
+    return BES_OK;
+  }
   const double target = im->rtol * bnorm;
   std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), y(m), hcol(m + 1);
   int total = 0;
@@ -929,14 +1003,14 @@ static int gmres(bes_ctx *c, Impl *im, int *iters_out, double *relres_out) {
   while (total < im->maxiter) {
     // r = b - A x (x = 0 on the first cycle, as the reference starts every solve)
     if (first) {
-      hipLaunchKernelGGL((k_scale<double, TV>), dim3(kRedBlocks), dim3(256), 0, c->stream, im->b, 1.0 / bnorm, V, n);
+      hipLaunchKernelGGL((k_scale<double, TV>), dim3(kRedBlocks), dim3(256), 0, c->stream, bb, 1.0 / bnorm, V, n);
       rnorm = bnorm;
     } else {
       // x in FP64 -> Z2 (TV), y = A Z2, r = b - y
-      hipLaunchKernelGGL((k_scale<double, TV>), dim3(kRedBlocks), dim3(256), 0, c->stream, im->x, 1.0, Z2, n);
+      hipLaunchKernelGGL((k_scale<double, TV>), dim3(kRedBlocks), dim3(256), 0, c->stream, xx, 1.0, Z2, n);
       int s = spmv<TM, TV>(c, im, Z2, Z1);
       if (s) return s;
-      hipLaunchKernelGGL((k_resid<TV>), dim3(kRedBlocks), dim3(256), 0, c->stream, im->b, Z1, Z, n, im->part);
+      hipLaunchKernelGGL((k_resid<TV>), dim3(kRedBlocks), dim3(256), 0, c->stream, bb, Z1, Z, n, im->part);
       hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(256), 0, c->stream, im->part, kRedBlocks, 1, im->dots);
       double r2;
       BES_HIP(hipMemcpyAsync(&r2, im->dots, sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -1012,22 +1086,25 @@ static int gmres(bes_ctx *c, Impl *im, int *iters_out, double *relres_out) {
     hipLaunchKernelGGL((k_combine<TV>), dim3(kRedBlocks), dim3(256), 0, c->stream, V, vs, j, im->dots, Z2, n);
     int s;
     if ((s = precond_apply<TM, TV>(c, im, Z2, Z, Z1))) return s;
-    hipLaunchKernelGGL((k_axpy_x<TV>), dim3(kRedBlocks), dim3(256), 0, c->stream, im->x, Z, n);
+    hipLaunchKernelGGL((k_axpy_x<TV>), dim3(kRedBlocks), dim3(256), 0, c->stream, xx, Z, n);
     BES_HIP(hipGetLastError());
     if (done) break;
   }
   // true residual of the returned solution
   {
-    hipLaunchKernelGGL((k_scale<double, TV>), dim3(kRedBlocks), dim3(256), 0, c->stream, im->x, 1.0, Z2, n);
+    hipLaunchKernelGGL((k_scale<double, TV>), dim3(kRedBlocks), dim3(256), 0, c->stream, xx, 1.0, Z2, n);
     int s = spmv<TM, TV>(c, im, Z2, Z1);
     if (s) return s;
-    hipLaunchKernelGGL((k_resid<TV>), dim3(kRedBlocks), dim3(256), 0, c->stream, im->b, Z1, Z, n, im->part);
+    hipLaunchKernelGGL((k_resid<TV>), dim3(kRedBlocks), dim3(256), 0, c->stream, bb, Z1, Z, n, im->part);
     hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(256), 0, c->stream, im->part, kRedBlocks, 1, im->dots);
     double r2;
     BES_HIP(hipMemcpyAsync(&r2, im->dots, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     BES_HIP(hipStreamSynchronize(c->stream));
     rnorm = std::sqrt(r2);
   }
+  if (colour)
+    hipLaunchKernelGGL(k_perm_rows<double>, dim3(1024), dim3(256), 0, c->stream, im->xc, im->crow, c->N, c->nv, 0,
+                       im->x);
   *iters_out = total;
   *relres_out = rnorm / bnorm;
   return BES_OK;

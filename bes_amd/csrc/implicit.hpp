@@ -39,6 +39,9 @@ struct Impl {
   double *b = nullptr;     // [N][nv] FP64 right-hand side
   double *x = nullptr;     // [N][nv] FP64 solution
   void *A = nullptr;       // matrix values (TM)
+  void *Ac = nullptr;      // A in colour order, for the ILU(0) solves (TM)
+  double *bc = nullptr;    // right-hand side in colour order
+  double *xc = nullptr;    // solution in colour order
   void *LU = nullptr;      // factor / inverse diagonal blocks (TM), [(k+1)*nv*nv][N]
   void *LM = nullptr;      // learned operator blocks (TM), [(k+1)*nv*nv][N] (learned.hip)
   void *lp = nullptr;      // learned-preconditioner network state (learned.hip)
