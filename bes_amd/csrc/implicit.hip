@@ -60,9 +60,11 @@ template <int ND>
 __global__ __launch_bounds__(256) void k_face_jacobian(const double *__restrict__ U, int N, FaceDev fd,
                                                        GasConst gc, FinfI fi, double *__restrict__ JF) {
   constexpr int NV = ND + 2, NB = NV * NV;
+  // face-fastest threads: lanes of a wave take consecutive faces of one input
+  // direction, so the entry-major stores below are coalesced
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  const int f = t / (2 * NV), dir = t % (2 * NV);
-  if (f >= fd.F) return;
+  const int dir = t / fd.F, f = t - dir * fd.F;
+  if (dir >= 2 * NV) return;
   const int o = fd.fo[f], j = fd.fn[f];
   if (j < 0 && dir >= NV) return;   // boundary faces: left Jacobian only
   double nn[ND];
@@ -83,9 +85,8 @@ __global__ __launch_bounds__(256) void k_face_jacobian(const double *__restrict_
     freestream_flux<ND, Dual>(gc, ul, ur, nn, F);
   }
   const int side = dir < NV ? 0 : 1, col = dir < NV ? dir : dir - NV;
-  double *J = JF + ((size_t)f * 2 + side) * NB;
 #pragma unroll
-  for (int r = 0; r < NV; r++) J[r * NV + col] = F[r].d;
+  for (int r = 0; r < NV; r++) JF[((size_t)side * NB + r * NV + col) * fd.F + f] = F[r].d;
 }
 
 // ---------------------------------------------------------------------------
@@ -138,18 +139,19 @@ __global__ __launch_bounds__(256) void k_assemble(
       const double mul = own ? mdt * Af : -(mdt * Af);
       // own row: diag += mul*JL, off(i,nbr) = mul*JR; neighbour row:
       // diag += (-m)*JR, off(i,owner) = (-m)*JL
-      const double *Jd = JF + ((size_t)f * 2 + (own ? 0 : 1)) * NB;
-      const double *Jo = JF + ((size_t)f * 2 + (own ? 1 : 0)) * NB;
+      const size_t FF_ = fd.F;
+      const double *Jd = JF + (size_t)(own ? 0 : 1) * NB * FF_ + f;   // entry e at Jd[e * F]
+      const double *Jo = JF + (size_t)(own ? 1 : 0) * NB * FF_ + f;
 #pragma unroll
       for (int e = 0; e < NB; e++) {
-        const double x = mul * Jd[e];
+        const double x = mul * Jd[(size_t)e * FF_];
         for (int r = 0; r < repeat; r++) D[e] += x;
       }
       TM *Ao = A + (size_t)(1 + m) * NB * N;
       if (j >= 0) {
 #pragma unroll
         for (int e = 0; e < NB; e++) {
-          const double x = mul * Jo[e];
+          const double x = mul * Jo[(size_t)e * FF_];
           double s = 0.0;
           for (int r = 0; r < repeat; r++) s += x;
           Ao[(size_t)e * N + i] = (TM)s;
@@ -700,20 +702,24 @@ static int impl_alloc(bes_ctx *c, Impl *im) {
   const Geometry &g = c->geo;
   const int N = c->N, k = c->k, nd = c->nd, nv = c->nv, NB = nv * nv;
   const int F = (int)g.nface;
-  // faces in reference order with device cell ids
+  // faces in device order (sorted by owner cell, as the flux pass) with device
+  // cell ids: a cell's Jacobian blocks sit next to its neighbours' in JF
   std::vector<int> fo(F), fn(F), cf((size_t)k * N), mrev((size_t)k * N, -1);
   std::vector<double> fnrm((size_t)nd * F);
-  for (int f = 0; f < F; f++) {
-    fo[f] = (int)c->iperm[g.owner[f]];
-    if (g.nbr[f] >= 0) fn[f] = (int)c->iperm[g.nbr[f]];
-    else fn[f] = (g.btype[f] == kSlipWall) ? -1 : -2;
-    for (int d = 0; d < nd; d++) fnrm[(size_t)d * F + f] = g.n[f * nd + d];
+  std::vector<int> ref2dev(F);
+  for (int q = 0; q < F; q++) ref2dev[c->fref[q]] = q;
+  for (int q = 0; q < F; q++) {
+    const int64_t f = c->fref[q];
+    fo[q] = (int)c->iperm[g.owner[f]];
+    if (g.nbr[f] >= 0) fn[q] = (int)c->iperm[g.nbr[f]];
+    else fn[q] = (g.btype[f] == kSlipWall) ? -1 : -2;
+    for (int d = 0; d < nd; d++) fnrm[(size_t)d * F + q] = g.n[f * nd + d];
   }
   for (int i = 0; i < N; i++) {
     const int64_t e = c->perm[i];
     for (int m = 0; m < k; m++) {
       const int64_t f = g.cface[e * k + m];
-      cf[(size_t)m * N + i] = (int)f;
+      cf[(size_t)m * N + i] = ref2dev[f];
       const int64_t j = g.cadj[e * k + m];
       if (j >= 0)
         for (int mm = 0; mm < k; mm++)
@@ -797,7 +803,8 @@ static int impl_alloc(bes_ctx *c, Impl *im) {
 static void impl_release(Impl *im) {
   lp::lp_release(im);
   void *ptrs[] = {im->fd.fo, im->fd.fn, im->fd.fnrm, im->fd.cf, im->fd.mrev, im->color, im->crow, im->JF,
-                  im->b, im->x, im->A, im->LU, im->LM, im->V, im->Z, im->part, im->dots, im->bits, im->Ac, im->bc, im->xc,
+                  im->b, im->x, im->A, im->LU, im->LM, im->LMh, im->V, im->Z, im->part, im->dots, im->bits, im->Ac,
+                  im->bc, im->xc,
                   im->cpos, im->cnb, im->cnp, im->clow};
   for (void *p : ptrs)
     if (p) hipFree(p);
@@ -881,10 +888,13 @@ static int precond_apply(bes_ctx *c, Impl *im, const TV *r, TV *z, TV *tmp) {
   // neighbour ids and colours; Jacobi: diagonal inverse, r, z
   ProfScope ps(c, kProfPrecond,
                im->precond == 2 ? blocks * sizeof(TM) + 4.0 * vb + 2.0 * N * (4.0 * c->k + 4.0)
-               : im->precond == 3 ? blocks * sizeof(TM) + 2.0 * vb + 4.0 * N * c->k
+               : im->precond == 3 ? blocks * (im->prec == 32 ? 2.0 : (double)sizeof(TM)) + 2.0 * vb + 4.0 * N * c->k
                : (im->precond == 1 ? (double)N * c->nv * c->nv * sizeof(TM) + 2.0 * vb : 2.0 * vb));
   if (im->precond == 0) {
     hipLaunchKernelGGL((k_scale<TV, TV>), dim3(kRedBlocks), dim3(256), 0, c->stream, r, 1.0, z, n);
+  } else if (im->precond == 3 && im->prec == 32) {   // learned operator, FP16 copy: one block SpMV
+    if (c->nd == 2) hipLaunchKernelGGL((k_spmv<2, __half, TV>), dim3(grid_for(N, bs)), dim3(bs), 0, c->stream, (const __half *)im->LMh, c->dm, r, z);
+    else hipLaunchKernelGGL((k_spmv<3, __half, TV>), dim3(grid_for(N, bs)), dim3(bs), 0, c->stream, (const __half *)im->LMh, c->dm, r, z);
   } else if (im->precond == 3) {   // learned operator: one block SpMV
     if (c->nd == 2) hipLaunchKernelGGL((k_spmv<2, TM, TV>), dim3(grid_for(N, bs)), dim3(bs), 0, c->stream, (const TM *)im->LM, c->dm, r, z);
     else hipLaunchKernelGGL((k_spmv<3, TM, TV>), dim3(grid_for(N, bs)), dim3(bs), 0, c->stream, (const TM *)im->LM, c->dm, r, z);

@@ -3,6 +3,7 @@
 // the block of the neighbour through local face m; entries stored [slot][e][N].
 #pragma once
 
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
 #include <vector>
@@ -14,12 +15,12 @@ namespace bes {
 
 constexpr int kRedBlocks = 1024;   // partial-sum blocks of the reductions
 
-struct FaceDev {
+struct FaceDev {          // faces in device order (the flux pass's), device cell ids
   int F = 0;
-  int *fo = nullptr;       // [F] owner (device order)
+  int *fo = nullptr;       // [F] owner
   int *fn = nullptr;       // [F] neighbour, -1 slip wall, -2 far field
   double *fnrm = nullptr;  // [nd][F] unit normal
-  int *cf = nullptr;       // [k][N] face id of (cell, local face)
+  int *cf = nullptr;       // [k][N] device face id of (cell, local face)
   int *mrev = nullptr;     // [k][N] local index of this face in the neighbour
 };
 
@@ -35,7 +36,7 @@ struct Impl {
   double rtol = 1e-7;
   int repeat_nv = 1;
   FaceDev fd;
-  double *JF = nullptr;    // [F][2][nv*nv]
+  double *JF = nullptr;    // [2][nv*nv][F] face Jacobian pairs, entry-major (device face order)
   double *b = nullptr;     // [N][nv] FP64 right-hand side
   double *x = nullptr;     // [N][nv] FP64 solution
   void *A = nullptr;       // matrix values (TM)
@@ -44,6 +45,8 @@ struct Impl {
   double *xc = nullptr;    // solution in colour order
   void *LU = nullptr;      // factor / inverse diagonal blocks (TM), [(k+1)*nv*nv][N]
   void *LM = nullptr;      // learned operator blocks (TM), [(k+1)*nv*nv][N] (learned.hip)
+  __half *LMh = nullptr;   // FP16 copy applied by the FP32 solves (the operator's precision
+                           // sets the convergence rate only, never the solution)
   void *lp = nullptr;      // learned-preconditioner network state (learned.hip)
   int lp_fallbacks = 0;    // solves that fell back to ILU(0)
   void *V = nullptr;       // Krylov basis (TV) [(m+1)][vstride]
@@ -129,6 +132,10 @@ __device__ __forceinline__ void store_block(TM *__restrict__ base, int slot, int
   for (int e = 0; e < NB; e++) base[((size_t)slot * NB + e) * N + i] = (TM)B[e];
 }
 
+__device__ __forceinline__ double to_d(double x) { return x; }
+__device__ __forceinline__ double to_d(float x) { return (double)x; }
+__device__ __forceinline__ double to_d(__half x) { return (double)__half2float(x); }
+
 // ---------------------------------------------------------------------------
 // y = A x for an ELL-BSR matrix (vectors [N][nv], type TV)
 // ---------------------------------------------------------------------------
@@ -147,7 +154,7 @@ __global__ __launch_bounds__(256) void k_spmv(const TM *__restrict__ A, DevMesh 
   for (int r = 0; r < NV; r++) {
     double s = 0;
 #pragma unroll
-    for (int c = 0; c < NV; c++) s += (double)A[((size_t)(r * NV + c)) * N + i] * xi[c];
+    for (int c = 0; c < NV; c++) s += to_d(A[((size_t)(r * NV + c)) * N + i]) * xi[c];
     acc[r] = s;
   }
 #pragma unroll
@@ -162,7 +169,7 @@ __global__ __launch_bounds__(256) void k_spmv(const TM *__restrict__ A, DevMesh 
     for (int r = 0; r < NV; r++) {
       double s = 0;
 #pragma unroll
-      for (int c = 0; c < NV; c++) s += (double)Ao[((size_t)(r * NV + c)) * N + i] * xj[c];
+      for (int c = 0; c < NV; c++) s += to_d(Ao[((size_t)(r * NV + c)) * N + i]) * xj[c];
       acc[r] += s;
     }
   }

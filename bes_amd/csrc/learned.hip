@@ -350,6 +350,12 @@ __global__ __launch_bounds__(256) void k_decode(const uint16_t *__restrict__ H, 
   }
 }
 
+// FP16 copy of the operator for the FP32 solves
+__global__ void k_to_half(const float *__restrict__ x, __half *__restrict__ y, size_t n) {
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
+    y[q] = __float2half(x[q]);
+}
+
 // ---------------------------------------------------------------------------
 // loss: Richardson chains and their adjoint (vectors [N][nv], float)
 // ---------------------------------------------------------------------------
@@ -977,7 +983,14 @@ static int lp_build_t(bes_ctx *c, Impl *im) {
 
 int lp_build(bes_ctx *c, Impl *im) {
   if (!im->lp) { set_error("learned preconditioner not initialised (bes_lp_init)"); return BES_ERR_ARG; }
-  return im->prec == 64 ? lp_build_t<double>(c, im) : lp_build_t<float>(c, im);
+  if (im->prec == 64) return lp_build_t<double>(c, im);
+  int s = lp_build_t<float>(c, im);
+  if (s) return s;
+  const size_t nm = (size_t)(c->k + 1) * c->nv * c->nv * c->N;
+  if (!im->LMh) BES_HIP(hipMalloc((void **)&im->LMh, nm * sizeof(__half)));
+  hipLaunchKernelGGL(k_to_half, dim3(2048), dim3(256), 0, c->stream, (const float *)im->LM, im->LMh, nm);
+  BES_HIP(hipGetLastError());
+  return BES_OK;
 }
 
 static double read_sumsq(bes_ctx *c, Lp *L, const float *x, size_t n, int *st) {
