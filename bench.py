@@ -427,7 +427,8 @@ def main():
     ap.add_argument("--quick-drop", action="store_true", help="skip the mixed FP32 ILU(0) drop run")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy 1 GiB device to device 5 times (calibrates HBM byte counters)")
-    ap.add_argument("--rtol", type=float, default=1e-3)
+    ap.add_argument("--rtol", type=float, default=5e-2,
+                    help="GMRES relative tolerance of every form (the outer loop needs no tighter)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-drop", action="store_true")

@@ -78,6 +78,8 @@ OPTIONAL_SIGS = {
     "bes_get_profile": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int],
     "bes_synchronize": [C.c_void_p],
     "bes_implicit_set_precond": [C.c_void_p, C.c_int],
+    "bes_write_solution": [C.c_void_p, C.c_char_p, C.c_int, C.c_int],
+    "bes_write_slip_wall": [C.c_void_p, C.c_char_p, C.c_int],
     "bes_implicit_set_reorth": [C.c_void_p, C.c_double],
     "bes_implicit_stats": [C.c_void_p, C.c_void_p, C.c_void_p],
     "bes_implicit_continue": [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p],
@@ -252,6 +254,14 @@ class Context:
         mx = np.zeros(self._shape(self.nv))
         _check(lib().bes_compute_residual(self.h, _p(R), _p(mx)), self.h)
         return R, mx
+
+    def write_solution(self, path, fmt="vtk", cond=0):
+        """Solution file in the reference's legacy-VTK or CSV format."""
+        _check(lib().bes_write_solution(self.h, str(path).encode(), 0 if fmt == "vtk" else 1, cond))
+
+    def write_slip_wall(self, path, cond=0):
+        """Slip-wall CSV in the reference's format."""
+        _check(lib().bes_write_slip_wall(self.h, str(path).encode(), cond))
 
     def set_profiling(self, on=True):
         _check(lib().bes_set_profiling(self.h, int(on)))

@@ -49,6 +49,8 @@ struct DevBsr {
 
 struct bes_ctx {
   bes::Geometry geo;                 // reference-order geometry (host)
+  std::vector<double> node_xyz;      // [nnode][nd] mesh nodes (solution files)
+  std::vector<int64_t> cell_nodes;   // [ncell][nd+1] cell vertices (solution files)
   std::vector<int64_t> perm, iperm;  // perm[new] = old cell, iperm[old] = new
   int nd = 0, nv = 0, k = 0, N = 0;
   bes::GasConst gc{};

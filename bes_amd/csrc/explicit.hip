@@ -567,6 +567,8 @@ static int create_from_input(const MeshInput &in, int geom_mode, const bes_gas *
   }
   if (c->geo.ncell > 0x7fffffff / 4) { set_error("mesh too large"); delete c; return BES_ERR_ARG; }
   c->nd = c->geo.nd; c->nv = c->nd + 2; c->k = c->geo.k; c->N = (int)c->geo.ncell;
+  c->node_xyz = in.coords;
+  c->cell_nodes = in.cells;
   c->gc = make_gas(gas);
   c->venkat_k = gas ? gas->venkat_k : 5.0;
   int s;

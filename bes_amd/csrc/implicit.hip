@@ -60,11 +60,13 @@ template <int ND>
 __global__ __launch_bounds__(256) void k_face_jacobian(const double *__restrict__ U, int N, FaceDev fd,
                                                        GasConst gc, FinfI fi, double *__restrict__ JF) {
   constexpr int NV = ND + 2, NB = NV * NV;
-  // face-fastest threads: lanes of a wave take consecutive faces of one input
-  // direction, so the entry-major stores below are coalesced
+  // groups of 64 faces x 2nv input directions: the lanes of a wave take 64
+  // consecutive faces of one direction (coalesced entry-major stores), and the
+  // group's waves reuse the same face states through the cache
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  const int dir = t / fd.F, f = t - dir * fd.F;
-  if (dir >= 2 * NV) return;
+  const int grp = t / (64 * 2 * NV), rem = t - grp * (64 * 2 * NV);
+  const int dir = rem >> 6, f = grp * 64 + (rem & 63);
+  if (f >= fd.F) return;
   const int o = fd.fo[f], j = fd.fn[f];
   if (j < 0 && dir >= NV) return;   // boundary faces: left Jacobian only
   double nn[ND];
@@ -822,7 +824,7 @@ static int launch_assemble(bes_ctx *c, Impl *im) {
     // per face: both states, normal, ids; writes one or two nv x nv blocks
     ProfScope ps(c, kProfJacobian, (double)im->fd.F * (16.0 * c->nv + 8.0 * c->nd + 8.0) +
                                        (double)(im->fd.F + im->ninternal) * nbl * 8.0);
-    const int nt = im->fd.F * 2 * c->nv;
+    const int nt = (im->fd.F + 63) / 64 * 64 * 2 * c->nv;   // whole groups of 64 faces
     if (c->nd == 2)
       hipLaunchKernelGGL(k_face_jacobian<2>, dim3(grid_for(nt, bs)), dim3(bs), 0, c->stream, c->U0, c->N, im->fd,
                          c->gc, fi, im->JF);

@@ -108,6 +108,8 @@ class Config {
   }
   const std::string &getMeshFilePath() const { return meshFile_; }
   const std::string &getOutputFolderPath() const { return outputFolder_; }
+  int getOutputFormat() const { return outputFormat_; }          // 0 VTK, 1 CSV
+  bool getOutputSlipWall() const { return outputSlipWall_; }
   short getNumDim() const { return (short)nDim_; }
   short getNumConservedVariable() const { return (short)nConserved_; }
   short getNumPrimitiveVariable() const { return (short)nPrimitive_; }
@@ -157,12 +159,14 @@ class Config {
   }
   static double num(const std::string &s) { return std::stod(s); }
   void apply_(const std::string &k, const std::vector<std::string> &a) {
-    static const char *ignored[] = {"MESH_FORMAT", "OUTPUT_FORMAT", "OUTPUT_SLIP_WALL", "EQUATION",
+    static const char *ignored[] = {"MESH_FORMAT", "EQUATION",
                                     "EXPLICIT_SCHEME", "GRADIENT_SCHEME", "LIMITER_SCHEME", "FLUX_SCHEME",
                                     "AMG_PRECOND_COARSENER", "AMG_PRECOND_SMOOTHER", "LINEAR_SOLVER"};
     for (const char *ig : ignored) if (k == ig) return;   // single scheme each on this path
     if (k == "MESH_FILE") meshFile_ = a[0];
     else if (k == "OUTPUT_FOLDER") outputFolder_ = a[0];
+    else if (k == "OUTPUT_FORMAT") outputFormat_ = a[0] == "CSV" ? 1 : 0;
+    else if (k == "OUTPUT_SLIP_WALL") outputSlipWall_ = a[0] == "YES";
     else if (k == "SOLVER_TYPE") solverType_ = a[0] == "IMPLICIT" ? SolverType::IMPLICIT : SolverType::EXPLICIT;
     else if (k == "MULTI_TIME_STEP_OPTIONS") {
       int n = (int)num(a[0]);
@@ -194,6 +198,8 @@ class Config {
     else throw Error(BES_ERR_ARG, "unknown config key " + k);
   }
   std::string meshFile_, outputFolder_ = ".";
+  int outputFormat_ = 0;
+  bool outputSlipWall_ = false;
   SolverType solverType_;
   LocalTimeStep localTimeStep_;
   unsigned long nRamp_ = 0, nTimeStep_ = 0, nSave_ = 1;
@@ -540,6 +546,21 @@ class Solver {
       for (int v = 0; v < nv; v++) all[e * nv + v] = U[v];
     check(bes_set_state(mesh_->ctx(), all.data()));
   }
+  // Solver.cpp:123-128 with Solution.cpp:129-161: solution_<n>.vtk|csv and
+  // slip_wall_<n>.csv after time step n (1-based), written every SAVE_INTERVAL
+  // steps and after the last (SolverExplicit.cpp:425-433, SolverImplicit.cpp:588-595)
+  void writeToFile_(unsigned long it) {
+    const unsigned long n = config_->getNumTimeStep(), save = std::max(1ul, config_->getNumTimeStepBetweenSave());
+    if (!(it % save == 0 || it == n - 1)) return;
+    const std::string base = config_->getOutputFolderPath() + "/";
+    const int fmt = config_->getOutputFormat();
+    const std::string sol = base + "solution_" + std::to_string(it + 1) + (fmt == 1 ? ".csv" : ".vtk");
+    check(bes_write_solution(mesh_->ctx(), sol.c_str(), fmt, 0));
+    if (config_->getOutputSlipWall()) {
+      const std::string sw = base + "slip_wall_" + std::to_string(it + 1) + ".csv";
+      check(bes_write_slip_wall(mesh_->ctx(), sw.c_str(), 0));
+    }
+  }
   // Solver.cpp:130-137: residual.csv rows
   void appendToResidual_(const std::string &file, const std::string &row) {
     std::ofstream f(file, std::ios::app);
@@ -555,7 +576,7 @@ class SolverExplicit : public Solver {
  public:
   void initialize(Config *config, Gas *gas, Mesh *mesh) override { initializeBase_(config, gas, mesh); }
   void setUniversalInitialCondition(double *primitive) override { uniformState_(primitive); }
-  // SolverExplicit.cpp:395-438 (freestream ramp and VTK output out of scope here)
+  // SolverExplicit.cpp:395-438 (the freestream ramp is out of scope here)
   void solve() override {
     const std::string file = config_->getOutputFolderPath() + "/residual.csv";
     { std::ofstream f(file); }
@@ -573,6 +594,7 @@ class SolverExplicit : public Solver {
       row << "\n";
       appendToResidual_(file, row.str());
       history_.push_back(linf);
+      writeToFile_(it);
     }
   }
 };
@@ -612,6 +634,7 @@ class SolverImplicit : public Solver {
       row << b << "\n";
       appendToResidual_(file, row.str());
       history_.push_back(std::vector<double>(r, r + 2 * nv + 3));
+      writeToFile_(it);
     }
   }
 };

@@ -226,6 +226,14 @@ int bes_lp_train(bes_ctx *ctx, int steps, double lr, int nprobe, int nrich, unsi
                  double *loss_hist);
 int bes_lp_fallbacks(bes_ctx *ctx);
 
+/* ---- solution files in the reference's formats, condition cond of the batch:
+ *      format 0 legacy VTK (SolutionSnapshot.cpp:172-266), 1 CSV of centroid and
+ *      primitives (:146-170); the slip-wall CSV of face centre, area vector and
+ *      owner primitives (:268-308).  Primitives are written in the reference's
+ *      dimensional form (GasPerfect.tpp:294-320). ---- */
+int bes_write_solution(bes_ctx *ctx, const char *path, int format, int cond);
+int bes_write_slip_wall(bes_ctx *ctx, const char *path, int cond);
+
 /* ---- measurement (no counterpart in the reference, which times nothing):
  *      when profiling is on, every kernel launch is bracketed by HIP events on
  *      the context's stream; bes_get_profile returns, per kernel kind, the

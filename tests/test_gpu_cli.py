@@ -2,7 +2,9 @@
 
 Writes a reference-format .besi and MSH2 mesh, runs ``bes_amd/bes_hip`` and
 compares residual.csv (the reference's format, SolverExplicit.cpp:380-390,
-SolverImplicit.cpp:328-343) with the oracle run on the same mesh.
+SolverImplicit.cpp:328-343) and the solution files written after the first and
+the last step (SolverExplicit.cpp:425-433, Solution.cpp:129-161) with the
+oracle run on the same mesh.
 """
 import os
 import subprocess
@@ -14,6 +16,8 @@ import scipy.sparse.linalg as spla
 
 from bes_amd import meshgen as mg
 from oracle import oracle as orc
+
+from test_gpu_output import dimensional, read_csv, read_vtk
 
 pytestmark = pytest.mark.gpu
 CLI = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "bes_amd", "bes_hip")
@@ -43,16 +47,25 @@ def test_cli_explicit_matches_oracle(tmp_path):
     got = _csv(tmp_path / "residual.csv")
     finf = orc.freestream_primitive((0.5, 0.05), 2)
     U = np.tile(orc.conserved_from_primitive(finf, 2), (md.ncell, 1))
-    _, linf = orc.Mesh(md).explicit(U, 6, 0.8, finf, nthreads=1)
+    Ufin, linf = orc.Mesh(md).explicit(U, 6, 0.8, finf, nthreads=1)
     assert got.shape == (6, 5)
     assert np.array_equal(got[:, 0], np.arange(1, 7))
     np.testing.assert_allclose(got[:, 1:], linf, rtol=6e-6, atol=1e-300)
+    # SAVE_INTERVAL larger than the run: files after the first and the last step
+    assert sorted(p.name for p in tmp_path.glob("solution_*")) == ["solution_1.vtk", "solution_6.vtk"]
+    v = read_vtk(tmp_path / "solution_6.vtk")
+    P = dimensional(Ufin, 2)
+    np.testing.assert_allclose(v["rho_nondim"], P[:, 0], rtol=6e-6)
+    np.testing.assert_allclose(v["P_nondim"], P[:, 3], rtol=6e-6)
+    np.testing.assert_allclose(v["M"][:, :2], P[:, 1:3], rtol=6e-6, atol=1e-6)
 
 
 def test_cli_implicit_matches_oracle(tmp_path):
     md = mg.bump_channel(24, 12, seed=10)
     res = _run(tmp_path, md, "IMPLICIT", 5, extra=["LINEAR_SOLVER_ABS_TOL = 1E-12",
-                                                   "LINEAR_SOLVER_MAX_ITER = 2000"])
+                                                   "LINEAR_SOLVER_MAX_ITER = 2000",
+                                                   "OUTPUT_FORMAT = CSV", "OUTPUT_SLIP_WALL = YES",
+                                                   "SAVE_INTERVAL = 2"])
     assert res.returncode == 0, res.stderr
     got = _csv(tmp_path / "residual.csv")
     # oracle: reference implicit loop with a direct solve and the adaptive CFL
@@ -72,6 +85,14 @@ def test_cli_implicit_matches_oracle(tmp_path):
     assert got.shape == (5, 1 + 4 + 2)
     np.testing.assert_allclose(got[:, 1:5], np.array(ref), rtol=1e-5)
     assert (got[:, 6] <= 1e-12 * 1.0001).all()
+    # SAVE_INTERVAL 2: after steps 1, 3 and 5 (0-based 0, 2, 4), CSV and slip wall
+    names = sorted(p.name for p in tmp_path.glob("s*_*.csv"))
+    assert names == ["slip_wall_1.csv", "slip_wall_3.csv", "slip_wall_5.csv",
+                     "solution_1.csv", "solution_3.csv", "solution_5.csv"]
+    sol = read_csv(tmp_path / "solution_5.csv")
+    np.testing.assert_allclose(sol[:, 2:], dimensional(U, 2), rtol=1e-5, atol=1e-7)
+    wall = read_csv(tmp_path / "slip_wall_5.csv")
+    assert wall.shape == ((om.faces()["btype"] == 2).sum(), 2 + 2 + 5)
 
 
 def test_cli_rejects_unknown_key(tmp_path):
