@@ -17,7 +17,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libbes_hip.so")
 CLI = os.path.join(HERE, "bes_hip")
-SOURCES = ["explicit.hip", "implicit.hip", "learned.hip", "geometry.cpp", "output.cpp"]
+SOURCES = ["explicit.hip", "implicit.hip", "learned.hip", "amg.hip", "geometry.cpp", "output.cpp"]
 HEADERS = ["context.hpp", "flux.hpp", "geometry.hpp", "kernels_common.hpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

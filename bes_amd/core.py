@@ -95,6 +95,10 @@ OPTIONAL_SIGS = {
     "bes_lp_loss_grad": [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p],
     "bes_lp_train": [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_int, C.c_ulonglong, C.c_void_p],
     "bes_lp_fallbacks": [C.c_void_p],
+    "bes_amg_set_params": [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int],
+    "bes_amg_info": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int],
+    "bes_amg_level": [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],
+    "bes_amg_apply": [C.c_void_p, C.c_void_p, C.c_void_p],
 }
 PROF_KINDS = ["timestep", "gradlim", "residual", "jacobian", "assemble", "spmv", "precond",
               "krylov", "factor", "update", "faceflux", "learned"]
@@ -280,7 +284,8 @@ class Context:
         _check(lib().bes_synchronize(self.h))
 
     # ---- implicit path ----
-    PRECOND = {"none": 0, "jacobi": 1, "ilu0": 2, "learned": 3}
+    PRECOND = {"none": 0, "jacobi": 1, "ilu0": 2, "learned": 3, "amg": 4}
+    AMG_RELAX = {"jacobi": 0, "spai0": 1, "block_jacobi": 2}
 
     def implicit_setup(self, precision=64, precond="ilu0", restart=30, maxiter=1000, rtol=1e-7):
         _check(lib().bes_implicit_setup(self.h, precision, self.PRECOND[precond], restart, maxiter, rtol))
@@ -376,6 +381,42 @@ class Context:
 
     def lp_fallbacks(self):
         return lib().bes_lp_fallbacks(self.h)
+
+    # ---- aggregation AMG preconditioner ----
+    def amg_set_params(self, eps_strong=0.08, over_interp=1.5, damping=0.72, coarse_enough=500, relax="jacobi"):
+        _check(lib().bes_amg_set_params(self.h, eps_strong, over_interp, damping, coarse_enough,
+                                        self.AMG_RELAX[relax]))
+
+    def amg_info(self):
+        """(rows per level, stored blocks per level, coarsest inverted)."""
+        nl = C.c_int(0)
+        direct = C.c_int(0)
+        rows = np.zeros(32, np.int64)
+        nnzb = np.zeros(32, np.int64)
+        _check(lib().bes_amg_info(self.h, C.byref(nl), _p(rows), _p(nnzb), C.byref(direct), 32))
+        return rows[:nl.value].copy(), nnzb[:nl.value].copy(), bool(direct.value)
+
+    def amg_level(self, level):
+        """Aggregates and states of the rows of a level, and for level >= 1 its
+        block CSR (rowptr, col, val [nnzb][nv][nv]) or None for level 0."""
+        rows, nnzb, _ = self.amg_info()
+        n = int(rows[level])
+        agg = np.zeros(n, np.int64)
+        st = np.zeros(n, np.uint8)
+        if level == 0:
+            _check(lib().bes_amg_level(self.h, 0, _p(agg), _p(st), None, None, None))
+            return agg, st, None
+        rp = np.zeros(n + 1, np.int64)
+        col = np.zeros(int(nnzb[level]), np.int64)
+        val = np.zeros((int(nnzb[level]), self.nv, self.nv))
+        _check(lib().bes_amg_level(self.h, level, _p(agg), _p(st), _p(rp), _p(col), _p(val)))
+        return agg, st, (rp, col, val)
+
+    def amg_apply(self, r):
+        r = np.ascontiguousarray(r, np.float64).reshape(self.ncell, self.nv)
+        z = np.zeros_like(r)
+        _check(lib().bes_amg_apply(self.h, _p(r), _p(z)))
+        return z
 
     def implicit_continue(self, nsteps):
         """nsteps more steps of the last implicit_run; returns (history, reached)."""

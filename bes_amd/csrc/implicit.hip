@@ -804,6 +804,7 @@ static int impl_alloc(bes_ctx *c, Impl *im) {
 
 static void impl_release(Impl *im) {
   lp::lp_release(im);
+  amg_release(im);
   void *ptrs[] = {im->fd.fo, im->fd.fn, im->fd.fnrm, im->fd.cf, im->fd.mrev, im->color, im->crow, im->JF,
                   im->b, im->x, im->A, im->LU, im->LM, im->LMh, im->V, im->Z, im->part, im->dots, im->bits, im->Ac,
                   im->bc, im->xc,
@@ -859,6 +860,7 @@ static int factor(bes_ctx *c, Impl *im) {
   const double used = (double)(N + 2 * im->ninternal) * nv * nv * sizeof(TM);   // stored blocks
   ProfScope ps(c, kProfFactor, im->precond == 1 ? 2.0 * N * nv * nv * sizeof(TM) : 3.0 * used);
   if (im->precond == 3) return lp::lp_build(c, im);
+  if (im->precond == 4) return amg_build(c, im);
   if (im->precond == 1) {
     if (c->nd == 2) hipLaunchKernelGGL((k_bj_setup<2, TM>), dim3(grid_for(N, bs)), dim3(bs), 0, c->stream, (const TM *)im->A, (TM *)im->LU, N);
     else hipLaunchKernelGGL((k_bj_setup<3, TM>), dim3(grid_for(N, bs)), dim3(bs), 0, c->stream, (const TM *)im->A, (TM *)im->LU, N);
@@ -883,6 +885,7 @@ static int factor(bes_ctx *c, Impl *im) {
 // z = M^-1 r
 template <typename TM, typename TV>
 static int precond_apply(bes_ctx *c, Impl *im, const TV *r, TV *z, TV *tmp) {
+  if (im->precond == 4) return amg_apply<TV>(c, im, r, z, tmp);
   const int bs = 256, N = c->N;
   const size_t n = (size_t)N * c->nv;
   const double vb = (double)n * sizeof(TV), blocks = (double)(N + 2 * im->ninternal) * c->nv * c->nv;
@@ -1136,9 +1139,9 @@ void bes_implicit_free(bes_ctx *c) {
 }
 
 int bes_implicit_setup(bes_ctx *c, int precision, int precond, int restart, int maxiter, double rtol) {
-  if (!c || (precision != 64 && precision != 32) || precond < 0 || precond > 3 || restart < 1 || restart > 30 ||
+  if (!c || (precision != 64 && precision != 32) || precond < 0 || precond > 4 || restart < 1 || restart > 30 ||
       maxiter < 1 || !(rtol > 0)) {
-    set_error("invalid implicit setup argument (precision 64/32, precond 0-3, restart 1-30)");
+    set_error("invalid implicit setup argument (precision 64/32, precond 0-4, restart 1-30)");
     return BES_ERR_ARG;
   }
   if (c->B != 1) {
@@ -1172,7 +1175,7 @@ int bes_implicit_stats(bes_ctx *c, long *reorths, int *lp_fallbacks) {
 }
 
 int bes_implicit_set_precond(bes_ctx *c, int precond) {
-  if (!c || !c->impl || precond < 0 || precond > 3) { set_error("call bes_implicit_setup first / precond 0-3"); return BES_ERR_ARG; }
+  if (!c || !c->impl || precond < 0 || precond > 4) { set_error("call bes_implicit_setup first / precond 0-4"); return BES_ERR_ARG; }
   Impl *im = impl_of(c);
   im->precond = precond;
   im->factored = false;
@@ -1354,6 +1357,40 @@ int bes_get_linear_solution(bes_ctx *c, double *x) {
   BES_HIP(hipMemcpy(h.data(), im->x, h.size() * sizeof(double), hipMemcpyDeviceToHost));
   for (int64_t e = 0; e < N; e++)
     for (int v = 0; v < nv; v++) x[e * nv + v] = h[(size_t)c->iperm[e] * nv + v];
+  return BES_OK;
+}
+
+int bes_amg_apply(bes_ctx *c, const double *r, double *z) {
+  if (!c || !c->impl || !r || !z) { set_error("call bes_implicit_setup first"); return BES_ERR_ARG; }
+  Impl *im = impl_of(c);
+  if (im->precond != 4 || !im->assembled) { set_error("AMG apply needs precond 4 and an assembled system"); return BES_ERR_ARG; }
+  int s;
+  if (!im->factored) {
+    if ((s = im->prec == 64 ? factor<double>(c, im) : factor<float>(c, im))) return s;
+    im->factored = true;
+  }
+  const int N = c->N, nv = c->nv;
+  const size_t n = (size_t)N * nv, vs = im->vstride;
+  std::vector<double> h(n);
+  for (int64_t e = 0; e < N; e++)
+    for (int v = 0; v < nv; v++) h[(size_t)c->iperm[e] * nv + v] = r[e * nv + v];
+  if (im->prec == 64) {
+    double *Z = (double *)im->Z;
+    BES_HIP(hipMemcpyAsync(Z, h.data(), n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    if ((s = amg_apply<double>(c, im, Z, Z + vs, Z + 2 * vs))) return s;
+    BES_HIP(hipMemcpyAsync(h.data(), Z + vs, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  } else {
+    float *Z = (float *)im->Z;
+    std::vector<float> hf(h.begin(), h.end());
+    BES_HIP(hipMemcpyAsync(Z, hf.data(), n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    if ((s = amg_apply<float>(c, im, Z, Z + vs, Z + 2 * vs))) return s;
+    BES_HIP(hipMemcpyAsync(hf.data(), Z + vs, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    BES_HIP(hipStreamSynchronize(c->stream));
+    std::copy(hf.begin(), hf.end(), h.begin());
+  }
+  BES_HIP(hipStreamSynchronize(c->stream));
+  for (int64_t e = 0; e < N; e++)
+    for (int v = 0; v < nv; v++) z[e * nv + v] = h[(size_t)c->iperm[e] * nv + v];
   return BES_OK;
 }
 

@@ -31,7 +31,7 @@ struct Mat {
 
 struct Impl {
   int prec = 64;           // 64 or 32
-  int precond = 2;         // 0 none, 1 block Jacobi, 2 multicolour ILU(0), 3 learned
+  int precond = 2;         // 0 none, 1 block Jacobi, 2 multicolour ILU(0), 3 learned, 4 AMG
   int restart = 30, maxiter = 1000;
   double rtol = 1e-7;
   int repeat_nv = 1;
@@ -49,6 +49,7 @@ struct Impl {
                            // sets the convergence rate only, never the solution)
   void *lp = nullptr;      // learned-preconditioner network state (learned.hip)
   int lp_fallbacks = 0;    // solves that fell back to ILU(0)
+  void *amg = nullptr;     // aggregation AMG hierarchy (amg.hip)
   void *V = nullptr;       // Krylov basis (TV) [(m+1)][vstride]
   void *Z = nullptr;       // work vectors (TV) [3][vstride]
   size_t vstride = 0;      // N*nv rounded up to 64 elements
@@ -78,6 +79,12 @@ struct Impl {
     std::vector<double> lb_old;
   } run;
 };
+
+// aggregation AMG (amg.hip): hierarchy from Impl::A, one V-cycle z = M r
+int amg_build(bes_ctx *c, Impl *im);
+template <typename TV>
+int amg_apply(bes_ctx *c, Impl *im, const TV *r, TV *z, TV *tmp);
+void amg_release(Impl *im);
 
 // ---------------------------------------------------------------------------
 // small dense block helpers (NV x NV, row-major, registers)

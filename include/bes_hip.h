@@ -146,13 +146,13 @@ int bes_jacobian_batch(int ndim, int kind, long n, const double *UL, const doubl
  *   precision 64 (FP64 matrix, factors and Krylov vectors) or 32 (FP32 inner
  *   solve; the right-hand side and the update stay FP64); precond 0 none,
  *   1 block Jacobi, 2 block ILU(0) in multicolour order, 3 the learned
- *   operator (bes_lp_*, below; falls back to ILU(0) for a solve it stalls);
- *   GMRES(restart),
+ *   operator (bes_lp_*, below; falls back to ILU(0) for a solve it stalls),
+ *   4 aggregation AMG (bes_amg_*, below); GMRES(restart),
  *   restart <= 30, stops at ||b - A x|| <= rtol ||b|| (the reference's
  *   LINEAR_SOLVER_ABS_TOL is this relative tolerance: CoefficientMatrix.cpp:304).
  *   Replaces CoefficientMatrix::initialize (CoefficientMatrix.cpp:114-164). */
 int bes_implicit_setup(bes_ctx *ctx, int precision, int precond, int restart, int maxiter, double rtol);
-/* switch the preconditioner (0-3) of an existing setup; the next solve factors again */
+/* switch the preconditioner (0-4) of an existing setup; the next solve factors again */
 int bes_implicit_set_precond(bes_ctx *ctx, int precond);
 /* classical Gram-Schmidt re-orthogonalises (DGKS) when the new Krylov vector
  * keeps less than eta of its norm (default 0.7; 0 never) */
@@ -225,6 +225,34 @@ int bes_lp_loss_grad(bes_ctx *ctx, const double *probes, int nprobe, int nrich, 
 int bes_lp_train(bes_ctx *ctx, int steps, double lr, int nprobe, int nrich, unsigned long long seed,
                  double *loss_hist);
 int bes_lp_fallbacks(bes_ctx *ctx);
+
+/* ---- aggregation AMG preconditioner (precond 4): the AMGCL hierarchy of
+ *      CoefficientMatrix.cpp:210-325 with coarsening "aggregation" and
+ *      aggr.block_size = nv (:231-256), rebuilt on the GPU for every solve after
+ *      an assembly.  Strength eps_strong on the largest |entry| of each block
+ *      (AMGCL's plain_aggregates / pointwise_aggregates); aggregates of radius
+ *      two around roots that are at least three strong links apart, chosen in
+ *      parallel rounds (the reference sweeps the rows in order); coarse
+ *      operators P^T A P / over_interp; levels while the scalar rows exceed
+ *      coarse_enough, the coarsest inverted densely; V-cycle from x = 0 with one
+ *      pre- and one post-sweep (amg.hpp:287-304, 498-541).
+ * bes_amg_set_params: eps_strong (0.08), over_interp (1.5), damping (0.72),
+ *   coarse_enough (500 scalar rows), relax 0 damped Jacobi on the scalar
+ *   diagonal (the reference's DAMPED_JACOBI), 1 SPAI0 (its SPAI0), 2 damped
+ *   block Jacobi on the nv x nv diagonal blocks (no reference counterpart).
+ * bes_amg_info: levels, block rows and stored blocks per level (up to maxlev),
+ *   direct = 1 when the coarsest level is inverted.
+ * bes_amg_level: the aggregate (next-level row, -1 removed) and state (1 root,
+ *   2 member, 3 removed) of every row of a level (level 0 in the reference's
+ *   cell order), and for levels >= 1 the block CSR of the level's matrix
+ *   (rowptr [n+1], col [nnzb] ascending, val [nnzb][nv][nv]); any may be NULL.
+ * bes_amg_apply: one V-cycle z = M r on host vectors [ncell][nv] (reference
+ *   cell order), building the hierarchy for the assembled system if needed. ---- */
+int bes_amg_set_params(bes_ctx *ctx, double eps_strong, double over_interp, double damping, int coarse_enough,
+                       int relax);
+int bes_amg_info(bes_ctx *ctx, int *nlevels, long *rows, long *nnzb, int *direct, int maxlev);
+int bes_amg_level(bes_ctx *ctx, int level, long *agg, unsigned char *state, long *rowptr, long *col, double *val);
+int bes_amg_apply(bes_ctx *ctx, const double *r, double *z);
 
 /* ---- solution files in the reference's formats, condition cond of the batch:
  *      format 0 legacy VTK (SolutionSnapshot.cpp:172-266), 1 CSV of centroid and
