@@ -55,9 +55,6 @@ enum State : unsigned char { kUndecided = 0, kRoot = 1, kMember = 2, kRemoved = 
 
 constexpr int kMaxLevels = 20;
 constexpr int kMaxRounds = 200;      // distance-2 independent-set rounds (O(log n) expected)
-constexpr int kHash = 2048;          // LDS hash table of coarse columns, per coarse row
-constexpr int kMaxCols = 1024;       // coarse columns per coarse row
-constexpr int kMaxCand = 2048;       // fine blocks gathered per coarse row
 constexpr int kDenseMax = 1024;      // largest coarsest system inverted densely (scalar rows)
 
 struct Params {
@@ -72,7 +69,7 @@ struct Level {
   // coarse levels: block CSR, ascending columns, diagonal position
   int *rp = nullptr, *col = nullptr, *dpos = nullptr;
   void *val = nullptr;      // [nnzb][nv*nv] (matrix type)
-  double *w = nullptr;      // relaxation: [n][nv] scalar factors or [n][nv*nv] damped inverse blocks
+  void *w = nullptr;        // relaxation (matrix type): [n][nv] scalar factors or [n][nv*nv] damped inverse blocks
   void *f = nullptr, *u = nullptr, *t = nullptr, *xa = nullptr;   // vectors [n][nv] (vector type)
   // coarsening to the next level
   int *agg = nullptr;       // [n] aggregate (= next-level row), -1 removed
@@ -98,7 +95,13 @@ struct Amg {
   std::vector<int *> scan_tmp;
   void *t0 = nullptr;       // level-0 residual (vector type)
   size_t vec_es = 0;
-  std::vector<void *> owned;
+  // FP32 solves: row-scaled FP16 copies of the level-0 matrix and relaxation
+  // data for the cycle (the hierarchy is built from the full-precision matrix)
+  __half *Ah = nullptr, *Wh = nullptr;
+  float *rm = nullptr;      // [N] row scale m_i: A_i = m_i Ah_i, W_i = Wh_i / m_i
+  std::vector<void *> owned;     // allocations outside the arena
+  void *arena = nullptr;
+  size_t arena_cap = 0, arena_off = 0, arena_need = 0;
 };
 
 static Amg *amg_of(Impl *im) { return (Amg *)im->amg; }
@@ -122,6 +125,21 @@ struct EllAcc {
   __device__ double vc(int cd, int e) const { return v(cd >> 3, cd & 7, e); }
   // entry of row j (a neighbour through entry q of row i) that points back to i
   __device__ int back(int i, int q, int) const { return 1 + mrev[(size_t)(q - 1) * N + i]; }
+};
+
+// level 0 from the row-scaled FP16 copy (cycle only)
+template <int NV>
+struct EllAccH {
+  static constexpr int NB = NV * NV;
+  const __half *A;
+  const float *rm;
+  const int *nbr;
+  int N, K;
+  __device__ int len(int) const { return K + 1; }
+  __device__ int col(int i, int q) const { return q == 0 ? i : nbr[(size_t)(q - 1) * N + i]; }
+  __device__ double v(int i, int q, int e) const {
+    return (double)__half2float(A[((size_t)q * NB + e) * N + i]) * (double)rm[i];
+  }
 };
 
 template <typename TM, int NV>
@@ -388,82 +406,96 @@ __global__ void k_scan_add(int *__restrict__ out, int n, const int *__restrict__
 // ---------------------------------------------------------------------------
 // Galerkin product, one wave per coarse row
 // ---------------------------------------------------------------------------
+template <int HASH>
 __device__ __forceinline__ int ht_insert(int *ht, int key, int *count) {
-  unsigned h = ((unsigned)key * 2654435761u) & (kHash - 1);
-  for (int probe = 0; probe < kHash; probe++) {
+  unsigned h = ((unsigned)key * 2654435761u) & (HASH - 1);
+  for (int probe = 0; probe < HASH; probe++) {
     const int old = atomicCAS(&ht[h], -1, key);
     if (old == -1) { atomicAdd(count, 1); return 1; }
     if (old == key) return 0;
-    h = (h + 1) & (kHash - 1);
+    h = (h + 1) & (HASH - 1);
   }
   return -1;
 }
 
-// gathers the coarse columns of row I into the LDS table; returns false on overflow
-template <typename Acc>
+// LDS sizes of the Galerkin kernels: a small configuration for rows with few
+// gathered blocks (most levels; high occupancy) and a large fallback
+struct GalSmall { static constexpr int HASH = 256, COLS = 128, CAND = 256, THREADS = 64; };
+struct GalMedium { static constexpr int HASH = 512, COLS = 256, CAND = 1024, THREADS = 128; };
+struct GalLarge { static constexpr int HASH = 2048, COLS = 1024, CAND = 2048, THREADS = 256; };
+
+// gathers the coarse columns of row I into the LDS table (one block per coarse
+// row); returns false when the row does not fit the configuration
+template <typename G, typename Acc>
 __device__ bool gather_cols(const Acc &a, int I, const int *__restrict__ mptr, const int *__restrict__ mem,
-                            const int *__restrict__ agg, int *ht, int *cnt, int *ncand, int *bad) {
-  const int lane = threadIdx.x;
-  for (int h = lane; h < kHash; h += 64) ht[h] = -1;
-  if (lane == 0) { *cnt = 0; *ncand = 0; *bad = 0; }
+                            const int *__restrict__ agg, int *ht, int *cnt, int *bad) {
+  const int t = threadIdx.x;
+  for (int h = t; h < G::HASH; h += G::THREADS) ht[h] = -1;
+  if (t == 0) { *cnt = 0; *bad = 0; }
   __syncthreads();
   const int m0 = mptr[I], m1 = mptr[I + 1];
   int off = 0;
   for (int mi = m0; mi < m1; mi++) {
     const int i = mem[mi];
     const int L = a.len(i);
-    for (int q = lane; q < L; q += 64) {
-      const int j = a.col(i, q);
-      const int J = j >= 0 ? agg[j] : -1;
-      if (J >= 0 && ht_insert(ht, J, cnt) < 0) atomicOr(bad, 1);
-    }
+    if (off + L <= G::CAND)
+      for (int q = t; q < L; q += G::THREADS) {
+        const int j = a.col(i, q);
+        const int J = j >= 0 ? agg[j] : -1;
+        if (J >= 0 && ht_insert<G::HASH>(ht, J, cnt) < 0) atomicOr(bad, 1);
+      }
     off += L;
   }
-  if (lane == 0) *ncand = off;
   __syncthreads();
-  return !(*bad) && *cnt <= kMaxCols && off <= kMaxCand && (*cnt) * 2 <= kHash;
+  return !(*bad) && *cnt <= G::COLS && off <= G::CAND && (*cnt) * 2 <= G::HASH;
 }
 
-template <typename Acc>
-__global__ __launch_bounds__(64) void k_gal_count(Acc a, int nc, const int *__restrict__ mptr,
-                                                  const int *__restrict__ mem, const int *__restrict__ agg,
-                                                  int *__restrict__ rowcnt, int *__restrict__ err) {
-  __shared__ int ht[kHash];
-  __shared__ int cnt, ncand, bad;
+// unique coarse columns per coarse row; err |= fail_bit when a row does not fit
+template <typename G, typename Acc>
+__global__ __launch_bounds__(G::THREADS) void k_gal_count(Acc a, int nc, const int *__restrict__ mptr,
+                                                          const int *__restrict__ mem, const int *__restrict__ agg,
+                                                          int *__restrict__ rowcnt, int *__restrict__ err,
+                                                          int fail_bit) {
+  __shared__ int ht[G::HASH];
+  __shared__ int cnt, bad;
   const int I = blockIdx.x;
   if (I >= nc) return;
-  const bool ok = gather_cols(a, I, mptr, mem, agg, ht, &cnt, &ncand, &bad);
+  const bool ok = gather_cols<G>(a, I, mptr, mem, agg, ht, &cnt, &bad);
   if (threadIdx.x == 0) {
     rowcnt[I] = cnt;
-    if (!ok) atomicOr(err, 2);
+    if (!ok) atomicOr(err, fail_bit);
   }
 }
 
-template <int NV, typename Acc, typename TMc>
-__global__ __launch_bounds__(64) void k_gal_fill(Acc a, int nc, const int *__restrict__ mptr,
-                                                 const int *__restrict__ mem, const int *__restrict__ agg,
-                                                 const int *__restrict__ rpc, int *__restrict__ colc,
-                                                 int *__restrict__ dposc, TMc *__restrict__ valc, double scale,
-                                                 int *__restrict__ err) {
-  constexpr int NB = NV * NV;
-  __shared__ int ht[kHash];
-  __shared__ int uj[kMaxCols], sj[kMaxCols];
-  __shared__ int ccode[kMaxCand], cslot[kMaxCand];
-  __shared__ int cnt, ncand, bad, nu;
-  const int I = blockIdx.x, lane = threadIdx.x;
+// Coarse row I: columns sorted ascending; the gathered fine blocks (members in
+// ascending order, entries in row order) are placed by a stable counting sort
+// on their coarse column, so that block (I, J) is the sum of its fine blocks
+// in that order.
+template <typename G, int NV, typename Acc, typename TMc>
+__global__ __launch_bounds__(G::THREADS) void k_gal_fill(Acc a, int nc, const int *__restrict__ mptr,
+                                                         const int *__restrict__ mem, const int *__restrict__ agg,
+                                                         const int *__restrict__ rpc, int *__restrict__ colc,
+                                                         int *__restrict__ dposc, TMc *__restrict__ valc,
+                                                         double scale, int *__restrict__ err) {
+  constexpr int NB = NV * NV, T = G::THREADS;
+  __shared__ int ht[G::HASH];
+  __shared__ int uj[G::COLS + 1], sj[G::COLS];
+  __shared__ int ccode[G::CAND], cslot[G::CAND], corder[G::CAND];
+  __shared__ int cnt, bad, nu;
+  const int I = blockIdx.x, t = threadIdx.x;
   if (I >= nc) return;
-  if (!gather_cols(a, I, mptr, mem, agg, ht, &cnt, &ncand, &bad)) {
-    if (lane == 0) atomicOr(err, 2);
+  if (!gather_cols<G>(a, I, mptr, mem, agg, ht, &cnt, &bad)) {
+    if (t == 0) atomicOr(err, 2);
     return;
   }
   // compact the table, then sort the columns by rank
-  if (lane == 0) nu = 0;
+  if (t == 0) nu = 0;
   __syncthreads();
-  for (int h = lane; h < kHash; h += 64)
+  for (int h = t; h < G::HASH; h += T)
     if (ht[h] >= 0) uj[atomicAdd(&nu, 1)] = ht[h];
   __syncthreads();
   const int u = nu;
-  for (int x = lane; x < u; x += 64) {
+  for (int x = t; x < u; x += T) {
     const int v = uj[x];
     int r = 0;
     for (int y = 0; y < u; y++) r += uj[y] < v;
@@ -471,7 +503,7 @@ __global__ __launch_bounds__(64) void k_gal_fill(Acc a, int nc, const int *__res
   }
   __syncthreads();
   const int base = rpc[I];
-  for (int r = lane; r < u; r += 64) {
+  for (int r = t; r < u; r += T) {
     colc[base + r] = sj[r];
     if (sj[r] == I) dposc[I] = r;
   }
@@ -481,7 +513,7 @@ __global__ __launch_bounds__(64) void k_gal_fill(Acc a, int nc, const int *__res
   for (int mi = m0; mi < m1; mi++) {
     const int i = mem[mi];
     const int L = a.len(i);
-    for (int q = lane; q < L; q += 64) {
+    for (int q = t; q < L; q += T) {
       const int j = a.col(i, q);
       const int J = j >= 0 ? agg[j] : -1;
       int slot = -1;
@@ -498,13 +530,31 @@ __global__ __launch_bounds__(64) void k_gal_fill(Acc a, int nc, const int *__res
     }
     off += L;
   }
+  // slot starts (uj reused): counts, then an exclusive scan by one thread
+  for (int r = t; r <= u; r += T) uj[r] = 0;
   __syncthreads();
-  // block (slot r, entry e): sum over candidates in order
-  for (int task = lane; task < u * NB; task += 64) {
+  for (int cc = t; cc < off; cc += T)
+    if (cslot[cc] >= 0) atomicAdd(&uj[cslot[cc]], 1);
+  __syncthreads();
+  if (t == 0) {
+    int acc = 0;
+    for (int r = 0; r <= u; r++) { const int v = uj[r]; uj[r] = acc; acc += v; }
+  }
+  __syncthreads();
+  // stable placement: rank among earlier candidates of the same slot
+  for (int cc = t; cc < off; cc += T) {
+    const int sl = cslot[cc];
+    if (sl < 0) continue;
+    int k = 0;
+    for (int y = 0; y < cc; y++) k += cslot[y] == sl;
+    corder[uj[sl] + k] = ccode[cc];
+  }
+  __syncthreads();
+  // block (slot r, entry e): its fine blocks in order
+  for (int task = t; task < u * NB; task += T) {
     const int r = task / NB, e = task - r * NB;
     double acc = 0;
-    for (int cc = 0; cc < off; cc++)
-      if (cslot[cc] == r) acc += a.vc(ccode[cc], e);
+    for (int p = uj[r]; p < uj[r + 1]; p++) acc += a.vc(corder[p], e);
     valc[(size_t)(base + r) * NB + e] = (TMc)(acc * scale);
   }
 }
@@ -512,8 +562,8 @@ __global__ __launch_bounds__(64) void k_gal_fill(Acc a, int nc, const int *__res
 // ---------------------------------------------------------------------------
 // relaxation set-up
 // ---------------------------------------------------------------------------
-template <int NV, int KIND, typename Acc>
-__global__ __launch_bounds__(256) void k_relax_setup(Acc a, int n, double damping, double *__restrict__ w,
+template <int NV, int KIND, typename Acc, typename TW>
+__global__ __launch_bounds__(256) void k_relax_setup(Acc a, int n, double damping, TW *__restrict__ w,
                                                      int *__restrict__ err) {
   constexpr int NB = NV * NV;
   const int i = row_index();
@@ -526,7 +576,7 @@ __global__ __launch_bounds__(256) void k_relax_setup(Acc a, int n, double dampin
     for (int e = 0; e < NB; e++) {
       const double x = damping * Inv[e];
       if (!isfinite(x)) atomicOr(err, 4);
-      w[(size_t)i * NB + e] = x;
+      w[(size_t)i * NB + e] = (TW)x;
     }
   } else {
     for (int r = 0; r < NV; r++) {
@@ -547,37 +597,43 @@ __global__ __launch_bounds__(256) void k_relax_setup(Acc a, int n, double dampin
         x = (1.0 / den) * arr;
       }
       if (!isfinite(x)) atomicOr(err, 4);
-      w[(size_t)i * NV + r] = x;
+      w[(size_t)i * NV + r] = (TW)x;
     }
   }
 }
 
 // ---------------------------------------------------------------------------
-// cycle kernels (vectors [n][nv])
+// cycle kernels (vectors [n][nv]; relaxation data in the matrix type TW)
 // ---------------------------------------------------------------------------
-template <int NV, int KIND>
-__device__ __forceinline__ void relax_apply(const double *__restrict__ w, int i, const double *t, double *d) {
+// d = W_i t; ws (FP16 copies) divides by the row scale
+template <int NV, int KIND, typename TW>
+__device__ __forceinline__ void relax_apply(const TW *__restrict__ w, const float *__restrict__ ws, int i,
+                                            const double *t, double *d) {
   if (KIND == kBlockJacobi) {
-    const double *W = w + (size_t)i * NV * NV;
+    const TW *W = w + (size_t)i * NV * NV;
     for (int r = 0; r < NV; r++) {
       double s = 0;
-      for (int c = 0; c < NV; c++) s += W[r * NV + c] * t[c];
+      for (int c = 0; c < NV; c++) s += to_d(W[r * NV + c]) * t[c];
       d[r] = s;
     }
   } else {
-    for (int r = 0; r < NV; r++) d[r] = w[(size_t)i * NV + r] * t[r];
+    for (int r = 0; r < NV; r++) d[r] = to_d(w[(size_t)i * NV + r]) * t[r];
+  }
+  if (ws) {
+    const double inv = 1.0 / (double)ws[i];
+    for (int r = 0; r < NV; r++) d[r] *= inv;
   }
 }
 
 // x = W f (one sweep from x = 0)
-template <int NV, int KIND, typename TV>
-__global__ __launch_bounds__(256) void k_relax0(int n, const double *__restrict__ w, const TV *__restrict__ f,
-                                                TV *__restrict__ x) {
+template <int NV, int KIND, typename TW, typename TV>
+__global__ __launch_bounds__(256) void k_relax0(int n, const TW *__restrict__ w, const float *__restrict__ ws,
+                                                const TV *__restrict__ f, TV *__restrict__ x) {
   const int i = row_index();
   if (i >= n) return;
   double t[NV], d[NV];
   for (int r = 0; r < NV; r++) t[r] = (double)f[(size_t)i * NV + r];
-  relax_apply<NV, KIND>(w, i, t, d);
+  relax_apply<NV, KIND>(w, ws, i, t, d);
   for (int r = 0; r < NV; r++) x[(size_t)i * NV + r] = (TV)d[r];
 }
 
@@ -599,6 +655,7 @@ __device__ __forceinline__ void row_resid(const Acc &a, int i, const TV *__restr
   }
 }
 
+// level 0 (ELL-BSR, entry-major): one thread per row, coalesced over rows
 template <int NV, typename Acc, typename TV>
 __global__ __launch_bounds__(256) void k_resid_lv(Acc a, int n, const TV *__restrict__ f, const TV *__restrict__ x,
                                                   TV *__restrict__ t) {
@@ -610,29 +667,117 @@ __global__ __launch_bounds__(256) void k_resid_lv(Acc a, int n, const TV *__rest
 }
 
 // xout = xin + W (f - A xin)
-template <int NV, int KIND, typename Acc, typename TV>
-__global__ __launch_bounds__(256) void k_relax(Acc a, int n, const double *__restrict__ w, const TV *__restrict__ f,
-                                               const TV *__restrict__ xin, TV *__restrict__ xout) {
+template <int NV, int KIND, typename Acc, typename TW, typename TV>
+__global__ __launch_bounds__(256) void k_relax(Acc a, int n, const TW *__restrict__ w, const float *__restrict__ ws,
+                                               const TV *__restrict__ f, const TV *__restrict__ xin,
+                                               TV *__restrict__ xout) {
   const int i = row_index();
   if (i >= n) return;
   double t[NV], d[NV];
   row_resid<NV>(a, i, f, xin, t);
-  relax_apply<NV, KIND>(w, i, t, d);
+  relax_apply<NV, KIND>(w, ws, i, t, d);
   for (int r = 0; r < NV; r++) xout[(size_t)i * NV + r] = (TV)((double)xin[(size_t)i * NV + r] + d[r]);
 }
 
+// row-scaled FP16 copies of the level-0 matrix and relaxation data: m_i is the
+// largest |entry| of the diagonal block; Ah = A_i / m_i, Wh = W_i m_i
+template <int NV, typename TM>
+__global__ __launch_bounds__(256) void k_half_copy(const TM *__restrict__ A, const TM *__restrict__ W, int N, int K,
+                                                   int wper, __half *__restrict__ Ah, __half *__restrict__ Wh,
+                                                   float *__restrict__ rm, int *__restrict__ err) {
+  constexpr int NB = NV * NV;
+  const int i = row_index();
+  if (i >= N) return;
+  double m = 0;
+  for (int e = 0; e < NB; e++) m = fmax(m, fabs(to_d(A[(size_t)e * N + i])));
+  if (!(m > 0)) m = 1.0;
+  const float mf = (float)m;
+  rm[i] = mf;
+  const double inv = 1.0 / (double)mf;
+  bool ok = true;
+  for (int q = 0; q <= K; q++)
+    for (int e = 0; e < NB; e++) {
+      const float v = (float)(to_d(A[((size_t)q * NB + e) * N + i]) * inv);
+      ok &= fabsf(v) < 65000.0f;
+      Ah[((size_t)q * NB + e) * N + i] = __float2half(v);
+    }
+  for (int e = 0; e < wper; e++) {
+    const float v = (float)(to_d(W[(size_t)i * wper + e]) * (double)mf);
+    ok &= fabsf(v) < 65000.0f;
+    Wh[(size_t)i * wper + e] = __float2half(v);
+  }
+  if (!ok) atomicOr(err, 16);
+}
+
+// coarse levels (block CSR, blocks contiguous): 32 lanes per row, lane e of
+// the group owns entry e of every block, so each block is one coalesced read.
+// Row sums over the nv lanes of a block row by shuffles.  RELAX = 0 writes
+// t = f - A x, RELAX = 1 writes x + W (f - A x).
+template <int NV, int KIND, int RELAX, typename TM, typename TV>
+__global__ __launch_bounds__(256) void k_csr_group(const int *__restrict__ rp, const int *__restrict__ cl,
+                                                   const TM *__restrict__ val, int n, const TM *__restrict__ w,
+                                                   const TV *__restrict__ f, const TV *__restrict__ x,
+                                                   TV *__restrict__ out) {
+  constexpr int NB = NV * NV;
+  static_assert(NB <= 32, "one block per 32-lane group");
+  const int i = (blockIdx.x * 256 + threadIdx.x) >> 5;
+  const int l = threadIdx.x & 31, gbase = threadIdx.x & 32;   // lane in group, group's first lane in the wave
+  const bool act = l < NB;
+  const int r = act ? l / NV : 0, cc = act ? l - r * NV : 0;
+  double acc = 0;
+  if (i < n) {
+    // the row's columns come in 32 at a time (one per lane) and reach the
+    // group by shuffles, so the block and x loads of a chunk are independent
+    const int p0 = rp[i], len = rp[i + 1] - p0;
+    for (int b0 = 0; b0 < len; b0 += 32) {
+      const int nb = min(32, len - b0);
+      const int Jl = l < nb ? cl[p0 + b0 + l] : 0;
+#pragma unroll 4
+      for (int q = 0; q < nb; q++) {
+        const int J = __shfl(Jl, gbase + q, 64);
+        if (act) acc += to_d(val[(size_t)(p0 + b0 + q) * NB + l]) * (double)x[(size_t)J * NV + cc];
+      }
+    }
+  }
+  // every lane of the wave takes part in the shuffles
+  double s = 0;
+  for (int c = 0; c < NV; c++) s += __shfl(acc, gbase + r * NV + c, 64);
+  double t = 0;
+  if (i < n && act && cc == 0) t = (double)f[(size_t)i * NV + r] - s;   // lane r*NV: row r of f - A x
+  if (RELAX == 0) {
+    if (i < n && act && cc == 0) out[(size_t)i * NV + r] = (TV)t;
+    return;
+  }
+  double d;
+  if (KIND == kBlockJacobi) {
+    // lane r*NV + c: W[r][c] * t_c, t_c from lane c*NV, then the row sum
+    const double tc = __shfl(t, gbase + cc * NV, 64);
+    const double prod = (i < n && act) ? to_d(w[(size_t)i * NB + l]) * tc : 0.0;
+    d = 0;
+    for (int c = 0; c < NV; c++) d += __shfl(prod, gbase + r * NV + c, 64);
+  } else {
+    d = (i < n && act && cc == 0) ? to_d(w[(size_t)i * NV + r]) * t : 0.0;
+  }
+  if (i < n && act && cc == 0) out[(size_t)i * NV + r] = (TV)((double)x[(size_t)i * NV + r] + d);
+}
+
+// fc[I] = sum of t over the members of I: 32 lanes per coarse row, lane m
+// takes members m, m + 32, ...; the lanes' sums meet in a fixed butterfly
 template <int NV, typename TV>
 __global__ __launch_bounds__(256) void k_restrict(int nc, const int *__restrict__ mptr, const int *__restrict__ mem,
                                                   const TV *__restrict__ t, TV *__restrict__ fc) {
-  const int I = row_index();
-  if (I >= nc) return;
+  const int I = (blockIdx.x * 256 + threadIdx.x) >> 5, l = threadIdx.x & 31;
   double s[NV];
   for (int r = 0; r < NV; r++) s[r] = 0;
-  for (int m = mptr[I]; m < mptr[I + 1]; m++) {
-    const int i = mem[m];
-    for (int r = 0; r < NV; r++) s[r] += (double)t[(size_t)i * NV + r];
-  }
-  for (int r = 0; r < NV; r++) fc[(size_t)I * NV + r] = (TV)s[r];
+  if (I < nc)
+    for (int m = mptr[I] + l; m < mptr[I + 1]; m += 32) {
+      const int i = mem[m];
+      for (int r = 0; r < NV; r++) s[r] += (double)t[(size_t)i * NV + r];
+    }
+  for (int o = 16; o > 0; o >>= 1)   // offsets below 32 stay inside the group
+    for (int r = 0; r < NV; r++) s[r] += __shfl_xor(s[r], o, 64);
+  if (I < nc && l == 0)
+    for (int r = 0; r < NV; r++) fc[(size_t)I * NV + r] = (TV)s[r];
 }
 
 template <int NV, typename TV>
@@ -734,8 +879,18 @@ __global__ void k_take_inverse(int m, const double *__restrict__ M, double *__re
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
+// Device memory of a build comes from one arena sized by the previous build's
+// total (first build: separate allocations), so the rebuild at every
+// nonlinear step allocates nothing once the sizes have settled.
 static int dmalloc(Amg *g, void **p, size_t bytes) {
-  BES_HIP(hipMalloc(p, std::max<size_t>(bytes, 16)));
+  bytes = (std::max<size_t>(bytes, 16) + 255) & ~(size_t)255;
+  g->arena_need += bytes;
+  if (g->arena_off + bytes <= g->arena_cap) {
+    *p = (char *)g->arena + g->arena_off;
+    g->arena_off += bytes;
+    return BES_OK;
+  }
+  BES_HIP(hipMalloc(p, bytes));
   g->owned.push_back(*p);
   return BES_OK;
 }
@@ -743,9 +898,12 @@ static int dmalloc(Amg *g, void **p, size_t bytes) {
 static void free_levels(Amg *g) {
   for (void *p : g->owned) hipFree(p);
   g->owned.clear();
+  g->arena_off = 0;
   g->lv.clear();
   g->scan_tmp.clear();
   g->dense = g->dense2 = g->inv = nullptr;
+  g->Ah = g->Wh = nullptr;
+  g->rm = nullptr;
   g->s = g->dia = nullptr;
   g->strong = g->sym = g->near = nullptr;
   g->m1 = nullptr;
@@ -861,15 +1019,33 @@ template <int NV, typename TM, typename Acc>
 static int galerkin(bes_ctx *c, Amg *g, int l, const Acc &a, int nc) {
   constexpr int NB = NV * NV;
   Level &L = g->lv[l], &C = g->lv[l + 1];
-  int s;
-  hipLaunchKernelGGL((k_gal_count<Acc>), dim3(nc), dim3(64), 0, c->stream, a, nc, L.mptr, L.mem, L.agg, g->ibuf,
-                     g->flags + 1);
+  int s, h[3];
+  // column counts with the small configuration; a level with a row that does
+  // not fit goes to the medium one, then to the large one
+  int cfg = 0;
+  hipLaunchKernelGGL((k_gal_count<GalSmall, Acc>), dim3(nc), dim3(GalSmall::THREADS), 0, c->stream, a, nc, L.mptr,
+                     L.mem, L.agg, g->ibuf, g->flags + 1, 8);
   BES_HIP(hipGetLastError());
+  if ((s = read_flags(c, g, h))) return s;
+  if (h[1] & 8) {
+    cfg = 1;
+    hipLaunchKernelGGL(k_fill_i, dim3(1), dim3(64), 0, c->stream, g->flags + 1, 1L, h[1] & ~8);
+    hipLaunchKernelGGL((k_gal_count<GalMedium, Acc>), dim3(nc), dim3(GalMedium::THREADS), 0, c->stream, a, nc,
+                       L.mptr, L.mem, L.agg, g->ibuf, g->flags + 1, 8);
+    BES_HIP(hipGetLastError());
+    if ((s = read_flags(c, g, h))) return s;
+    if (h[1] & 8) {
+      cfg = 2;
+      hipLaunchKernelGGL(k_fill_i, dim3(1), dim3(64), 0, c->stream, g->flags + 1, 1L, h[1] & ~8);
+      hipLaunchKernelGGL((k_gal_count<GalLarge, Acc>), dim3(nc), dim3(GalLarge::THREADS), 0, c->stream, a, nc,
+                         L.mptr, L.mem, L.agg, g->ibuf, g->flags + 1, 2);
+      BES_HIP(hipGetLastError());
+    }
+  }
   if ((s = dmalloc(g, (void **)&C.rp, (size_t)(nc + 1) * sizeof(int)))) return s;
   hipLaunchKernelGGL(k_fill_i, dim3(1), dim3(64), 0, c->stream, g->ibuf + nc, 1L, 0);
   long nnzb = 0;
   if ((s = scan_ex(c, g, g->ibuf, C.rp, nc + 1, 0, &nnzb))) return s;
-  int h[3];
   if ((s = read_flags(c, g, h))) return s;
   if (h[1] & 2) { set_error("AMG coarse row exceeds the gather limits"); return BES_ERR_SOLVE; }
   C.n = nc;
@@ -880,28 +1056,36 @@ static int galerkin(bes_ctx *c, Amg *g, int l, const Acc &a, int nc) {
     return s;
   hipLaunchKernelGGL(k_fill_i, dim3(256), dim3(256), 0, c->stream, C.dpos, (long)nc, 0);
   const double scale = (double)(1.0f / (float)g->prm.over_interp);
-  hipLaunchKernelGGL((k_gal_fill<NV, Acc, TM>), dim3(nc), dim3(64), 0, c->stream, a, nc, L.mptr, L.mem, L.agg, C.rp,
-                     C.col, C.dpos, (TM *)C.val, scale, g->flags + 1);
+  if (cfg == 2)
+    hipLaunchKernelGGL((k_gal_fill<GalLarge, NV, Acc, TM>), dim3(nc), dim3(GalLarge::THREADS), 0, c->stream, a, nc,
+                       L.mptr, L.mem, L.agg, C.rp, C.col, C.dpos, (TM *)C.val, scale, g->flags + 1);
+  else if (cfg == 1)
+    hipLaunchKernelGGL((k_gal_fill<GalMedium, NV, Acc, TM>), dim3(nc), dim3(GalMedium::THREADS), 0, c->stream, a,
+                       nc, L.mptr, L.mem, L.agg, C.rp, C.col, C.dpos, (TM *)C.val, scale, g->flags + 1);
+  else
+    hipLaunchKernelGGL((k_gal_fill<GalSmall, NV, Acc, TM>), dim3(nc), dim3(GalSmall::THREADS), 0, c->stream, a, nc,
+                       L.mptr, L.mem, L.agg, C.rp, C.col, C.dpos, (TM *)C.val, scale, g->flags + 1);
   BES_HIP(hipGetLastError());
   return BES_OK;
 }
 
-template <int NV, typename Acc>
+template <int NV, typename TM, typename Acc>
 static int relax_setup(bes_ctx *c, Amg *g, int l, const Acc &a) {
   Level &L = g->lv[l];
   const int n = L.n, bs = 256, nb = blocks_for(n, bs);
   const int per = g->prm.relax == kBlockJacobi ? NV * NV : NV;
-  int s = dmalloc(g, (void **)&L.w, (size_t)n * per * sizeof(double));
+  int s = dmalloc(g, &L.w, (size_t)n * per * sizeof(TM));
   if (s) return s;
+  TM *w = (TM *)L.w;
   if (g->prm.relax == kBlockJacobi)
-    hipLaunchKernelGGL((k_relax_setup<NV, kBlockJacobi, Acc>), dim3(nb), dim3(bs), 0, c->stream, a, n,
-                       g->prm.damping, L.w, g->flags + 1);
+    hipLaunchKernelGGL((k_relax_setup<NV, kBlockJacobi, Acc, TM>), dim3(nb), dim3(bs), 0, c->stream, a, n,
+                       g->prm.damping, w, g->flags + 1);
   else if (g->prm.relax == kSpai0)
-    hipLaunchKernelGGL((k_relax_setup<NV, kSpai0, Acc>), dim3(nb), dim3(bs), 0, c->stream, a, n, g->prm.damping,
-                       L.w, g->flags + 1);
+    hipLaunchKernelGGL((k_relax_setup<NV, kSpai0, Acc, TM>), dim3(nb), dim3(bs), 0, c->stream, a, n, g->prm.damping,
+                       w, g->flags + 1);
   else
-    hipLaunchKernelGGL((k_relax_setup<NV, kJacobi, Acc>), dim3(nb), dim3(bs), 0, c->stream, a, n, g->prm.damping,
-                       L.w, g->flags + 1);
+    hipLaunchKernelGGL((k_relax_setup<NV, kJacobi, Acc, TM>), dim3(nb), dim3(bs), 0, c->stream, a, n, g->prm.damping,
+                       w, g->flags + 1);
   BES_HIP(hipGetLastError());
   return BES_OK;
 }
@@ -921,6 +1105,15 @@ static int build_t(bes_ctx *c, Impl *im, Amg *g) {
   const size_t es = im->prec == 32 ? sizeof(float) : sizeof(double);
   if (N >= (1 << 28)) { set_error("AMG: at most 2^28 cells"); return BES_ERR_ARG; }
   free_levels(g);
+  if (g->arena_need > g->arena_cap) {
+    if (g->arena) BES_HIP(hipFree(g->arena));
+    g->arena = nullptr;
+    g->arena_cap = 0;
+    const size_t cap = g->arena_need + g->arena_need / 8;
+    BES_HIP(hipMalloc(&g->arena, cap));
+    g->arena_cap = cap;
+  }
+  g->arena_need = 0;
   g->vec_es = es;
   int s;
   const size_t ent0 = (size_t)(K + 1) * N;
@@ -947,8 +1140,18 @@ static int build_t(bes_ctx *c, Impl *im, Amg *g) {
       return s;
     if ((s = dmalloc(g, (void **)&L.agg, n * sizeof(int))) || (s = dmalloc(g, (void **)&L.state, n)))
       return s;
-    s = l == 0 ? relax_setup<NV>(c, g, l, ell<NV, TM>(c, im)) : relax_setup<NV>(c, g, l, csr<NV, TM>(L));
+    s = l == 0 ? relax_setup<NV, TM>(c, g, l, ell<NV, TM>(c, im)) : relax_setup<NV, TM>(c, g, l, csr<NV, TM>(L));
     if (s) return s;
+    if (l == 0 && im->prec == 32) {
+      const int wper = g->prm.relax == kBlockJacobi ? NV * NV : NV;
+      if ((s = dmalloc(g, (void **)&g->Ah, (size_t)(K + 1) * NV * NV * N * sizeof(__half))) ||
+          (s = dmalloc(g, (void **)&g->Wh, (size_t)N * wper * sizeof(__half))) ||
+          (s = dmalloc(g, (void **)&g->rm, (size_t)N * sizeof(float))))
+        return s;
+      hipLaunchKernelGGL((k_half_copy<NV, TM>), dim3(blocks_for(N, 256)), dim3(256), 0, c->stream,
+                         (const TM *)im->A, (const TM *)L.w, N, K, wper, g->Ah, g->Wh, g->rm, g->flags + 1);
+      BES_HIP(hipGetLastError());
+    }
     const long rows = (long)n * NV;
     if (rows <= g->prm.coarse_enough) {   // coarsest: dense inverse
       g->direct = rows <= kDenseMax;
@@ -996,6 +1199,7 @@ static int build_t(bes_ctx *c, Impl *im, Amg *g) {
   int h[3];
   if ((s = read_flags(c, g, h))) return s;
   if (h[1] & 4) { set_error("AMG relaxation has a singular diagonal"); return BES_ERR_SOLVE; }
+  if (h[1] & 16) g->Ah = g->Wh = nullptr, g->rm = nullptr;   // out of FP16 range: full-precision level 0
   if (h[2]) { set_error("AMG coarsest system is singular"); return BES_ERR_SOLVE; }
   g->built_type = (int)sizeof(TM);
   return BES_OK;
@@ -1004,54 +1208,87 @@ static int build_t(bes_ctx *c, Impl *im, Amg *g) {
 // ---------------------------------------------------------------------------
 // V-cycle
 // ---------------------------------------------------------------------------
-template <int NV, typename TV, typename Acc>
-static void launch_relax0(bes_ctx *c, int kind, int n, const double *w, const TV *f, TV *x) {
-  const int nb = blocks_for(n, 256);
-  if (kind == kBlockJacobi) hipLaunchKernelGGL((k_relax0<NV, kBlockJacobi, TV>), dim3(nb), dim3(256), 0, c->stream, n, w, f, x);
-  else hipLaunchKernelGGL((k_relax0<NV, kJacobi, TV>), dim3(nb), dim3(256), 0, c->stream, n, w, f, x);
-}
-template <int NV, typename TV, typename Acc>
-static void launch_relax(bes_ctx *c, int kind, const Acc &a, int n, const double *w, const TV *f, const TV *xin,
-                         TV *xout) {
+template <int NV, typename TW, typename TV>
+static void launch_relax0_t(bes_ctx *c, int kind, int n, const TW *w, const float *ws, const TV *f, TV *x) {
   const int nb = blocks_for(n, 256);
   if (kind == kBlockJacobi)
-    hipLaunchKernelGGL((k_relax<NV, kBlockJacobi, Acc, TV>), dim3(nb), dim3(256), 0, c->stream, a, n, w, f, xin, xout);
+    hipLaunchKernelGGL((k_relax0<NV, kBlockJacobi, TW, TV>), dim3(nb), dim3(256), 0, c->stream, n, w, ws, f, x);
   else
-    hipLaunchKernelGGL((k_relax<NV, kJacobi, Acc, TV>), dim3(nb), dim3(256), 0, c->stream, a, n, w, f, xin, xout);
+    hipLaunchKernelGGL((k_relax0<NV, kJacobi, TW, TV>), dim3(nb), dim3(256), 0, c->stream, n, w, ws, f, x);
 }
 
-template <int NV, typename TM, typename TV, typename Acc>
-static void cycle_level(bes_ctx *c, Impl *im, Amg *g, int l, const Acc &a, const TV *f, TV *x, TV *xa, TV *t);
+// x = W f on level l (level 0 of an FP32 solve from the FP16 copies)
+template <int NV, typename TM, typename TV>
+static void launch_relax0(bes_ctx *c, Amg *g, int l, const TV *f, TV *x) {
+  const Level &L = g->lv[l];
+  if (l == 0 && g->Ah) launch_relax0_t<NV, __half, TV>(c, g->prm.relax, L.n, g->Wh, g->rm, f, x);
+  else launch_relax0_t<NV, TM, TV>(c, g->prm.relax, L.n, (const TM *)L.w, nullptr, f, x);
+}
+
+template <int NV, typename Acc, typename TW, typename TV>
+static void launch_ell_sweep(bes_ctx *c, int kind, const Acc &a, int n, bool relax, const TW *w, const float *ws,
+                             const TV *f, const TV *x, TV *out) {
+  const int nb = blocks_for(n, 256);
+  if (!relax)
+    hipLaunchKernelGGL((k_resid_lv<NV, Acc, TV>), dim3(nb), dim3(256), 0, c->stream, a, n, f, x, out);
+  else if (kind == kBlockJacobi)
+    hipLaunchKernelGGL((k_relax<NV, kBlockJacobi, Acc, TW, TV>), dim3(nb), dim3(256), 0, c->stream, a, n, w, ws, f,
+                       x, out);
+  else
+    hipLaunchKernelGGL((k_relax<NV, kJacobi, Acc, TW, TV>), dim3(nb), dim3(256), 0, c->stream, a, n, w, ws, f, x,
+                       out);
+}
+
+// t = f - A x (relax = false) or xout = x + W (f - A x) (relax = true) on level l
+template <int NV, typename TM, typename TV>
+static void launch_sweep(bes_ctx *c, Impl *im, Amg *g, int l, bool relax, const TV *f, const TV *x, TV *out) {
+  const Level &L = g->lv[l];
+  const int n = L.n, kind = g->prm.relax;
+  const TM *w = (const TM *)L.w;
+  if (l == 0) {
+    if (g->Ah)
+      launch_ell_sweep<NV>(c, kind, EllAccH<NV>{g->Ah, g->rm, c->dm.nbr, c->N, c->k}, n, relax,
+                           (const __half *)g->Wh, (const float *)g->rm, f, x, out);
+    else
+      launch_ell_sweep<NV>(c, kind, ell<NV, TM>(c, im), n, relax, w, (const float *)nullptr, f, x, out);
+    return;
+  }
+  const int nb = blocks_for((long)n * 32, 256);
+  const TM *val = (const TM *)L.val;
+  if (!relax)
+    hipLaunchKernelGGL((k_csr_group<NV, kJacobi, 0, TM, TV>), dim3(nb), dim3(256), 0, c->stream, L.rp, L.col, val, n,
+                       w, f, x, out);
+  else if (kind == kBlockJacobi)
+    hipLaunchKernelGGL((k_csr_group<NV, kBlockJacobi, 1, TM, TV>), dim3(nb), dim3(256), 0, c->stream, L.rp, L.col,
+                       val, n, w, f, x, out);
+  else
+    hipLaunchKernelGGL((k_csr_group<NV, kJacobi, 1, TM, TV>), dim3(nb), dim3(256), 0, c->stream, L.rp, L.col, val, n,
+                       w, f, x, out);
+}
 
 template <int NV, typename TM, typename TV>
 static void cycle(bes_ctx *c, Impl *im, Amg *g, int l, const TV *f, TV *x, TV *xa, TV *t) {
-  if (l == 0) cycle_level<NV, TM, TV>(c, im, g, l, ell<NV, TM>(c, im), f, x, xa, t);
-  else cycle_level<NV, TM, TV>(c, im, g, l, csr<NV, TM>(g->lv[l]), f, x, xa, t);
-}
-
-template <int NV, typename TM, typename TV, typename Acc>
-static void cycle_level(bes_ctx *c, Impl *im, Amg *g, int l, const Acc &a, const TV *f, TV *x, TV *xa, TV *t) {
   const Level &L = g->lv[l];
-  const int n = L.n, kind = g->prm.relax;   // SPAI0 and damped Jacobi share the scalar kernels
+  const int n = L.n;   // SPAI0 and damped Jacobi share the scalar kernels
   const int nb = blocks_for(n, 256);
   if (l == g->nlev - 1) {
     if (g->direct) {
       hipLaunchKernelGGL((k_dense_apply<TV>), dim3(blocks_for(g->ndense, 4)), dim3(256), 0, c->stream, g->ndense,
                          g->inv, f, x);
     } else {
-      launch_relax0<NV, TV, Acc>(c, kind, n, L.w, f, xa);
-      launch_relax<NV, TV>(c, kind, a, n, L.w, f, xa, x);
+      launch_relax0<NV, TM, TV>(c, g, l, f, xa);
+      launch_sweep<NV, TM, TV>(c, im, g, l, true, f, xa, x);
     }
     return;
   }
   const Level &C = g->lv[l + 1];
-  launch_relax0<NV, TV, Acc>(c, kind, n, L.w, f, xa);
-  hipLaunchKernelGGL((k_resid_lv<NV, Acc, TV>), dim3(nb), dim3(256), 0, c->stream, a, n, f, xa, t);
-  hipLaunchKernelGGL((k_restrict<NV, TV>), dim3(blocks_for(C.n, 256)), dim3(256), 0, c->stream, C.n, L.mptr, L.mem,
-                     t, (TV *)C.f);
+  launch_relax0<NV, TM, TV>(c, g, l, f, xa);
+  launch_sweep<NV, TM, TV>(c, im, g, l, false, f, xa, t);
+  hipLaunchKernelGGL((k_restrict<NV, TV>), dim3(blocks_for((long)C.n * 32, 256)), dim3(256), 0, c->stream, C.n,
+                     L.mptr, L.mem, t, (TV *)C.f);
   cycle<NV, TM, TV>(c, im, g, l + 1, (const TV *)C.f, (TV *)C.u, (TV *)C.xa, (TV *)C.t);
   hipLaunchKernelGGL((k_prolong<NV, TV>), dim3(nb), dim3(256), 0, c->stream, n, L.agg, (const TV *)C.u, xa);
-  launch_relax<NV, TV>(c, kind, a, n, L.w, f, xa, x);
+  launch_sweep<NV, TM, TV>(c, im, g, l, true, f, xa, x);
 }
 
 static double level_bytes(const Amg *g, int nv, size_t es_m, size_t es_v) {
@@ -1061,7 +1298,8 @@ static double level_bytes(const Amg *g, int nv, size_t es_m, size_t es_v) {
   for (int l = 0; l < g->nlev; l++) {
     const Level &L = g->lv[l];
     const double per = g->prm.relax == kBlockJacobi ? nv * nv : nv;
-    b += 2.0 * L.nnzb * nv * nv * es_m + 2.0 * L.n * per * 8.0 + 8.0 * L.n * nv * es_v;
+    const double em = (l == 0 && g->Ah) ? 2.0 : (double)es_m;
+    b += 2.0 * L.nnzb * nv * nv * em + 2.0 * L.n * per * em + 8.0 * L.n * nv * es_v;
   }
   if (g->direct) b += (double)g->ndense * g->ndense * 8.0;
   return b;
@@ -1101,6 +1339,7 @@ template int amg_apply<float>(bes_ctx *, Impl *, const float *, float *, float *
 void amg_release(Impl *im) {
   if (!im->amg) return;
   free_levels(amg_of(im));
+  if (amg_of(im)->arena) hipFree(amg_of(im)->arena);
   delete amg_of(im);
   im->amg = nullptr;
 }

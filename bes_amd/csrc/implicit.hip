@@ -855,12 +855,12 @@ static int launch_assemble(bes_ctx *c, Impl *im) {
 
 template <typename TM>
 static int factor(bes_ctx *c, Impl *im) {
+  if (im->precond == 4) return amg_build(c, im);   // times itself
   const int bs = 256, N = c->N, nv = c->nv;
   const size_t mat = (size_t)(c->k + 1) * nv * nv * N;
   const double used = (double)(N + 2 * im->ninternal) * nv * nv * sizeof(TM);   // stored blocks
   ProfScope ps(c, kProfFactor, im->precond == 1 ? 2.0 * N * nv * nv * sizeof(TM) : 3.0 * used);
   if (im->precond == 3) return lp::lp_build(c, im);
-  if (im->precond == 4) return amg_build(c, im);
   if (im->precond == 1) {
     if (c->nd == 2) hipLaunchKernelGGL((k_bj_setup<2, TM>), dim3(grid_for(N, bs)), dim3(bs), 0, c->stream, (const TM *)im->A, (TM *)im->LU, N);
     else hipLaunchKernelGGL((k_bj_setup<3, TM>), dim3(grid_for(N, bs)), dim3(bs), 0, c->stream, (const TM *)im->A, (TM *)im->LU, N);

@@ -138,6 +138,8 @@ class Config {
   int getGeometryMode() const { return geomMode_; }
   int getPrecision() const { return precision_; }
   int getPreconditioner() const { return precond_; }
+  bool getAmgAggregation() const { return amgAggregation_; }
+  int getAmgRelax() const { return amgRelax_; }   // 0 damped Jacobi, 1 SPAI0, 2 block Jacobi, -1 other
   int getBlockRepeat() const { return blockRepeat_; }
 
  private:
@@ -161,7 +163,7 @@ class Config {
   void apply_(const std::string &k, const std::vector<std::string> &a) {
     static const char *ignored[] = {"MESH_FORMAT", "EQUATION",
                                     "EXPLICIT_SCHEME", "GRADIENT_SCHEME", "LIMITER_SCHEME", "FLUX_SCHEME",
-                                    "AMG_PRECOND_COARSENER", "AMG_PRECOND_SMOOTHER", "LINEAR_SOLVER"};
+                                    "LINEAR_SOLVER"};
     for (const char *ig : ignored) if (k == ig) return;   // single scheme each on this path
     if (k == "MESH_FILE") meshFile_ = a[0];
     else if (k == "OUTPUT_FOLDER") outputFolder_ = a[0];
@@ -193,7 +195,11 @@ class Config {
     else if (k == "GEOMETRY_MODE") geomMode_ = a[0] == "EXACT" ? BES_GEOM_EXACT : BES_GEOM_COMPAT;
     else if (k == "LINEAR_SOLVER_PRECISION") precision_ = (int)num(a[0]);
     else if (k == "PRECONDITIONER")
-      precond_ = a[0] == "NONE" ? 0 : a[0] == "BLOCK_JACOBI" ? 1 : a[0] == "LEARNED" ? 3 : 2;
+      precond_ = a[0] == "NONE" ? 0 : a[0] == "BLOCK_JACOBI" ? 1 : a[0] == "LEARNED" ? 3 : a[0] == "AMG" ? 4 : 2;
+    // Config.cpp:365-400; used when PRECONDITIONER = AMG
+    else if (k == "AMG_PRECOND_COARSENER") amgAggregation_ = a[0] == "AGGREGATION";
+    else if (k == "AMG_PRECOND_SMOOTHER")
+      amgRelax_ = a[0] == "DAMPED_JACOBI" ? 0 : a[0] == "SPAI0" ? 1 : a[0] == "BLOCK_JACOBI" ? 2 : -1;
     else if (k == "JACOBIAN_BLOCK_REPEAT") blockRepeat_ = a[0] == "NO" ? 0 : 1;
     else throw Error(BES_ERR_ARG, "unknown config key " + k);
   }
@@ -209,6 +215,8 @@ class Config {
   double linTol_ = 1e-7, venkat_ = 5.0, harten_ = 0.05, R_ = 287, gamma_ = 1.4;
   int nDim_ = 0, nConserved_ = 0, nPrimitive_ = 0, nCurrentSolution_ = 1;
   int geomMode_ = BES_GEOM_COMPAT, precision_ = 64, precond_ = 2, blockRepeat_ = 1;
+  bool amgAggregation_ = true;
+  int amgRelax_ = 0;
 };
 
 // ---------------------------------------------------------------------------
@@ -447,6 +455,21 @@ typedef FluxJacobianFreestream FluxJacobianFreestreamThreeDim;
 typedef FluxJacobianSlipWall FluxJacobianSlipWallTwoDim;
 typedef FluxJacobianSlipWall FluxJacobianSlipWallThreeDim;
 
+// preconditioner state after bes_implicit_setup: the learned operator's
+// initial weights, or the AMG parameters of the reference's test
+// configurations (eps_strong 0.08, over_interp 1.5, damping 0.72, coarse_enough
+// 500: CoefficientMatrix.cpp:231-280 with AMGCL's defaults)
+inline void setupPreconditioner_(bes_ctx *ctx, const Config *config) {
+  if (config->getPreconditioner() == 3) check(bes_lp_init(ctx, 20261202ull));
+  if (config->getPreconditioner() == 4) {
+    if (!config->getAmgAggregation())
+      throw Error(BES_ERR_ARG, "AMG_PRECOND_COARSENER: only AGGREGATION is available");
+    if (config->getAmgRelax() < 0)
+      throw Error(BES_ERR_ARG, "AMG_PRECOND_SMOOTHER: DAMPED_JACOBI, SPAI0 or BLOCK_JACOBI");
+    check(bes_amg_set_params(ctx, 0.08, 1.5, 0.72, 500, config->getAmgRelax()));
+  }
+}
+
 // ---------------------------------------------------------------------------
 // CoefficientMatrix: host CSR in the reference layout, solved on the GPU
 // ---------------------------------------------------------------------------
@@ -458,7 +481,7 @@ class CoefficientMatrix {
     nv_ = config->getNumConservedVariable();
     check(bes_implicit_setup(ctx_, config->getPrecision(), config->getPreconditioner(), 30,
                              config->getLinearSolverNumIter(), config->getLinearSolverTol()));
-    if (config->getPreconditioner() == 3) check(bes_lp_init(ctx_, 20261202ull));
+    setupPreconditioner_(ctx_, config);
     const long nnz = bes_implicit_nnz(ctx_);
     const long nrow = (long)mesh->getNumElement() * nv_;
     row_.assign(nrow + 1, 0);
@@ -605,7 +628,7 @@ class SolverImplicit : public Solver {
     initializeBase_(config, gas, mesh);
     check(bes_implicit_setup(mesh->ctx(), config->getPrecision(), config->getPreconditioner(), 30,
                              config->getLinearSolverNumIter(), config->getLinearSolverTol()));
-    if (config->getPreconditioner() == 3) check(bes_lp_init(mesh->ctx(), 20261202ull));
+    setupPreconditioner_(mesh->ctx(), config);
   }
   void setUniversalInitialCondition(double *primitive) override { uniformState_(primitive); }
   // SolverImplicit.cpp:552-600 with the adaptive CFL of :530-550

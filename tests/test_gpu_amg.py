@@ -123,6 +123,22 @@ def test_cycle_relaxations_match_restatement(relax, kind):
         assert np.abs(z - zr).max() <= 1e-9 * np.abs(zr).max()
 
 
+def test_fp32_cycle_close_to_restatement():
+    """FP32 solves run the level-0 sweeps from row-scaled FP16 copies of the
+    matrix and relaxation blocks; the cycle stays within that rounding of the
+    FP64 restatement built from the same aggregates."""
+    c, A, b = _context((8, 4, 6), precision=32, relax="block_jacobi")
+    with c:
+        rng = np.random.default_rng(9)
+        r = rng.standard_normal(b.shape)
+        z = c.amg_apply(r).ravel()
+        _, _, levels, inv = _oracle_from_gpu(c, A, "block")
+        zr = amg.vcycle(levels, inv, r)
+        rel = np.linalg.norm(z - zr) / np.linalg.norm(zr)
+        print("FP32 cycle rel err %.2e" % rel)
+        assert rel < 1e-2
+
+
 def test_gmres_amg_solves_and_compares_with_row_sweep():
     c, A, b = _context((16, 8, 12), cfl=20.0, relax="block_jacobi", rtol=1e-8, repeat=False)
     with c:

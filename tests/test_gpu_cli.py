@@ -60,12 +60,14 @@ def test_cli_explicit_matches_oracle(tmp_path):
     np.testing.assert_allclose(v["M"][:, :2], P[:, 1:3], rtol=6e-6, atol=1e-6)
 
 
-def test_cli_implicit_matches_oracle(tmp_path):
+@pytest.mark.parametrize("precond", [[], ["PRECONDITIONER = AMG", "AMG_PRECOND_COARSENER = AGGREGATION",
+                                            "AMG_PRECOND_SMOOTHER = BLOCK_JACOBI"]])
+def test_cli_implicit_matches_oracle(tmp_path, precond):
     md = mg.bump_channel(24, 12, seed=10)
     res = _run(tmp_path, md, "IMPLICIT", 5, extra=["LINEAR_SOLVER_ABS_TOL = 1E-12",
                                                    "LINEAR_SOLVER_MAX_ITER = 2000",
                                                    "OUTPUT_FORMAT = CSV", "OUTPUT_SLIP_WALL = YES",
-                                                   "SAVE_INTERVAL = 2"])
+                                                   "SAVE_INTERVAL = 2"] + precond)
     assert res.returncode == 0, res.stderr
     got = _csv(tmp_path / "residual.csv")
     # oracle: reference implicit loop with a direct solve and the adaptive CFL
@@ -93,6 +95,12 @@ def test_cli_implicit_matches_oracle(tmp_path):
     np.testing.assert_allclose(sol[:, 2:], dimensional(U, 2), rtol=1e-5, atol=1e-7)
     wall = read_csv(tmp_path / "slip_wall_5.csv")
     assert wall.shape == ((om.faces()["btype"] == 2).sum(), 2 + 2 + 5)
+
+
+def test_cli_rejects_unavailable_amg_smoother(tmp_path):
+    md = mg.tri_patch(3, 3)
+    res = _run(tmp_path, md, "IMPLICIT", 1, extra=["PRECONDITIONER = AMG", "AMG_PRECOND_SMOOTHER = GAUSS_SEIDEL"])
+    assert res.returncode == 1 and "AMG_PRECOND_SMOOTHER" in res.stderr + res.stdout
 
 
 def test_cli_rejects_unknown_key(tmp_path):
