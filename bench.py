@@ -113,7 +113,7 @@ def implicit_workload(args):
             if precond == "learned":
                 c.lp_init(20261202)
             if precond == "amg":
-                c.amg_set_params(relax="block_jacobi")
+                c.amg_set_params(relax="block_jacobi", rebuild_interval=args.amg_rebuild)
 
         def forces():
             U = c.get_state()
@@ -185,9 +185,10 @@ def implicit_workload(args):
                 "learned_preconditioner": "network with its initial weights (blend 0.953 between block "
                                           "Jacobi and the first-order Neumann operator); Adam training on "
                                           "the GPU lowers its loss but not yet its GMRES iterations",
-                "amg_preconditioner": "aggregation AMG rebuilt every step (eps_strong 0.08, over_interp 1.5, "
-                                      "coarsest <= 500 rows inverted), V-cycle with damped (0.72) block "
-                                      "Jacobi relaxation; level-0 sweeps from row-scaled FP16 copies"}
+                "amg_preconditioner": "aggregation AMG (eps_strong 0.08, over_interp 1.5, coarsest <= 500 "
+                                      "rows inverted) rebuilt every %d solves, V-cycle with damped (0.72) "
+                                      "block Jacobi relaxation; sweeps from row-scaled FP16 copies"
+                                      % args.amg_rebuild}
     cfgd = {"workload": "implicit-wing-2m" if args.wing == [96, 48, 72] else "implicit-wing",
             "mesh": f"swept tapered wing (LE sweep 30 deg, taper 0.56, AR 3.8, 10% closed-TE 4-digit "
                     f"section), half-span box, far field 10 chords, {args.wing[0]}x{args.wing[1]}x"
@@ -433,6 +434,8 @@ def main():
     ap.add_argument("--quick-drop", action="store_true", help="skip the mixed FP32 ILU(0) drop run")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy 1 GiB device to device 5 times (calibrates HBM byte counters)")
+    ap.add_argument("--amg-rebuild", type=int, default=6,
+                    help="AMG hierarchy rebuilt at every n-th solve of the mixed AMG run")
     ap.add_argument("--rtol", type=float, default=5e-2,
                     help="GMRES relative tolerance of every form (the outer loop needs no tighter)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)

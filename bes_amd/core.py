@@ -95,8 +95,8 @@ OPTIONAL_SIGS = {
     "bes_lp_loss_grad": [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p],
     "bes_lp_train": [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_int, C.c_ulonglong, C.c_void_p],
     "bes_lp_fallbacks": [C.c_void_p],
-    "bes_amg_set_params": [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int],
-    "bes_amg_info": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int],
+    "bes_amg_set_params": [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int],
+    "bes_amg_info": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int],
     "bes_amg_level": [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],
     "bes_amg_apply": [C.c_void_p, C.c_void_p, C.c_void_p],
 }
@@ -383,18 +383,21 @@ class Context:
         return lib().bes_lp_fallbacks(self.h)
 
     # ---- aggregation AMG preconditioner ----
-    def amg_set_params(self, eps_strong=0.08, over_interp=1.5, damping=0.72, coarse_enough=500, relax="jacobi"):
+    def amg_set_params(self, eps_strong=0.08, over_interp=1.5, damping=0.72, coarse_enough=500, relax="jacobi",
+                       rebuild_interval=1):
         _check(lib().bes_amg_set_params(self.h, eps_strong, over_interp, damping, coarse_enough,
-                                        self.AMG_RELAX[relax]))
+                                        self.AMG_RELAX[relax], rebuild_interval))
 
-    def amg_info(self):
-        """(rows per level, stored blocks per level, coarsest inverted)."""
+    def amg_info(self, with_builds=False):
+        """(rows per level, stored blocks per level, coarsest inverted[, builds so far])."""
         nl = C.c_int(0)
         direct = C.c_int(0)
+        builds = C.c_long(0)
         rows = np.zeros(32, np.int64)
         nnzb = np.zeros(32, np.int64)
-        _check(lib().bes_amg_info(self.h, C.byref(nl), _p(rows), _p(nnzb), C.byref(direct), 32))
-        return rows[:nl.value].copy(), nnzb[:nl.value].copy(), bool(direct.value)
+        _check(lib().bes_amg_info(self.h, C.byref(nl), _p(rows), _p(nnzb), C.byref(direct), C.byref(builds), 32))
+        out = (rows[:nl.value].copy(), nnzb[:nl.value].copy(), bool(direct.value))
+        return out + (builds.value,) if with_builds else out
 
     def amg_level(self, level):
         """Aggregates and states of the rows of a level, and for level >= 1 its

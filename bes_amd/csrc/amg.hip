@@ -61,6 +61,7 @@ struct Params {
   double eps_strong = 0.08, over_interp = 1.5, damping = 0.72;
   int coarse_enough = 500;
   int relax = kJacobi;
+  int rebuild = 1;          // hierarchy rebuilt at every rebuild-th solve after an assembly
 };
 
 struct Level {
@@ -76,11 +77,17 @@ struct Level {
   unsigned char *state = nullptr;   // [n] final State of the row
   int *mptr = nullptr, *mem = nullptr;   // members of next-level row I: mem[mptr[I] .. mptr[I+1]), ascending
   int ncoarse = 0;
+  // FP32 solves, coarse levels: row-scaled FP16 copy of val for the sweeps
+  __half *hval = nullptr;
+  float *hrm = nullptr;
 };
 
 struct Amg {
   Params prm;
   int built_type = 0;       // sizeof(TM) of the built hierarchy, 0 none
+  long builds = 0;          // hierarchies built so far
+  int reused = 0;           // solves since the last build
+  bool force = true;        // next request builds (new parameters)
   std::vector<Level> lv;
   int nlev = 0;
   bool direct = false;
@@ -713,11 +720,13 @@ __global__ __launch_bounds__(256) void k_half_copy(const TM *__restrict__ A, con
 // the group owns entry e of every block, so each block is one coalesced read.
 // Row sums over the nv lanes of a block row by shuffles.  RELAX = 0 writes
 // t = f - A x, RELAX = 1 writes x + W (f - A x).
-template <int NV, int KIND, int RELAX, typename TM, typename TV>
+// TS is the value type of the sweep copy (the matrix type, or FP16 with row
+// scales rm: A_i = rm_i * val_i)
+template <int NV, int KIND, int RELAX, typename TS, typename TM, typename TV>
 __global__ __launch_bounds__(256) void k_csr_group(const int *__restrict__ rp, const int *__restrict__ cl,
-                                                   const TM *__restrict__ val, int n, const TM *__restrict__ w,
-                                                   const TV *__restrict__ f, const TV *__restrict__ x,
-                                                   TV *__restrict__ out) {
+                                                   const TS *__restrict__ val, const float *__restrict__ rm, int n,
+                                                   const TM *__restrict__ w, const TV *__restrict__ f,
+                                                   const TV *__restrict__ x, TV *__restrict__ out) {
   constexpr int NB = NV * NV;
   static_assert(NB <= 32, "one block per 32-lane group");
   const int i = (blockIdx.x * 256 + threadIdx.x) >> 5;
@@ -742,6 +751,7 @@ __global__ __launch_bounds__(256) void k_csr_group(const int *__restrict__ rp, c
   // every lane of the wave takes part in the shuffles
   double s = 0;
   for (int c = 0; c < NV; c++) s += __shfl(acc, gbase + r * NV + c, 64);
+  if (rm && i < n) s *= (double)rm[i];
   double t = 0;
   if (i < n && act && cc == 0) t = (double)f[(size_t)i * NV + r] - s;   // lane r*NV: row r of f - A x
   if (RELAX == 0) {
@@ -761,23 +771,48 @@ __global__ __launch_bounds__(256) void k_csr_group(const int *__restrict__ rp, c
   if (i < n && act && cc == 0) out[(size_t)i * NV + r] = (TV)((double)x[(size_t)i * NV + r] + d);
 }
 
-// fc[I] = sum of t over the members of I: 32 lanes per coarse row, lane m
-// takes members m, m + 32, ...; the lanes' sums meet in a fixed butterfly
-template <int NV, typename TV>
+// fc[I] = sum of t over the members of I: G lanes per coarse row (G a power
+// of two <= 32, about the mean aggregate size), lane m takes members m, m + G,
+// ...; the lanes' sums meet in a fixed butterfly
+template <int NV, int G, typename TV>
 __global__ __launch_bounds__(256) void k_restrict(int nc, const int *__restrict__ mptr, const int *__restrict__ mem,
                                                   const TV *__restrict__ t, TV *__restrict__ fc) {
-  const int I = (blockIdx.x * 256 + threadIdx.x) >> 5, l = threadIdx.x & 31;
+  const int I = (blockIdx.x * 256 + threadIdx.x) / G, l = threadIdx.x & (G - 1);
   double s[NV];
   for (int r = 0; r < NV; r++) s[r] = 0;
   if (I < nc)
-    for (int m = mptr[I] + l; m < mptr[I + 1]; m += 32) {
+    for (int m = mptr[I] + l; m < mptr[I + 1]; m += G) {
       const int i = mem[m];
       for (int r = 0; r < NV; r++) s[r] += (double)t[(size_t)i * NV + r];
     }
-  for (int o = 16; o > 0; o >>= 1)   // offsets below 32 stay inside the group
+  for (int o = G / 2; o > 0; o >>= 1)   // offsets below G stay inside the group
     for (int r = 0; r < NV; r++) s[r] += __shfl_xor(s[r], o, 64);
   if (I < nc && l == 0)
     for (int r = 0; r < NV; r++) fc[(size_t)I * NV + r] = (TV)s[r];
+}
+
+// row-scaled FP16 sweep copy of a coarse level: rm_i = largest |entry| of the
+// diagonal block, hval = val / rm_i (err |= 64 when out of FP16 range)
+template <int NV, typename TM>
+__global__ __launch_bounds__(256) void k_half_csr(int n, const int *__restrict__ rp, const int *__restrict__ dpos,
+                                                  const TM *__restrict__ val, __half *__restrict__ hval,
+                                                  float *__restrict__ rm, int *__restrict__ err) {
+  constexpr int NB = NV * NV;
+  const int i = row_index();
+  if (i >= n) return;
+  double m = 0;
+  for (int e = 0; e < NB; e++) m = fmax(m, fabs(to_d(val[(size_t)(rp[i] + dpos[i]) * NB + e])));
+  if (!(m > 0)) m = 1.0;
+  const float mf = (float)m;
+  rm[i] = mf;
+  const double inv = 1.0 / (double)mf;
+  bool ok = true;
+  for (size_t k = (size_t)rp[i] * NB; k < (size_t)rp[i + 1] * NB; k++) {
+    const float v = (float)(to_d(val[k]) * inv);
+    ok &= fabsf(v) < 65000.0f;
+    hval[k] = __float2half(v);
+  }
+  if (!ok) atomicOr(err, 64);
 }
 
 template <int NV, typename TV>
@@ -1173,6 +1208,23 @@ static int build_t(bes_ctx *c, Impl *im, Amg *g) {
     if (s) return s;
   }
   g->nlev = (int)g->lv.size();
+  if (im->prec == 32)
+    for (int l = 1; l < g->nlev; l++) {
+      if (l == g->nlev - 1 && g->direct) break;   // the coarsest is applied as a dense inverse
+      Level &Lh = g->lv[l];
+      if ((s = dmalloc(g, (void **)&Lh.hval, (size_t)Lh.nnzb * NV * NV * sizeof(__half))) ||
+          (s = dmalloc(g, (void **)&Lh.hrm, (size_t)Lh.n * sizeof(float))))
+        return s;
+      hipLaunchKernelGGL((k_half_csr<NV, TM>), dim3(blocks_for(Lh.n, 256)), dim3(256), 0, c->stream, Lh.n, Lh.rp,
+                         Lh.dpos, (const TM *)Lh.val, Lh.hval, Lh.hrm, g->flags + 1);
+      int hf[3];
+      if ((s = read_flags(c, g, hf))) return s;
+      if (hf[1] & 64) {   // out of FP16 range: this level sweeps its full-precision values
+        Lh.hval = nullptr;
+        Lh.hrm = nullptr;
+        hipLaunchKernelGGL(k_fill_i, dim3(1), dim3(64), 0, c->stream, g->flags + 1, 1L, hf[1] & ~64);
+      }
+    }
   if (g->direct) {
     Level &Cl = g->lv.back();
     const int m = Cl.n * NV;
@@ -1239,6 +1291,21 @@ static void launch_ell_sweep(bes_ctx *c, int kind, const Acc &a, int n, bool rel
                        out);
 }
 
+template <int NV, typename TS, typename TM, typename TV>
+static void launch_csr_sweep(bes_ctx *c, int kind, const Level &L, bool relax, const TS *val, const float *rm,
+                             const TM *w, const TV *f, const TV *x, TV *out) {
+  const int n = L.n, nb = blocks_for((long)n * 32, 256);
+  if (!relax)
+    hipLaunchKernelGGL((k_csr_group<NV, kJacobi, 0, TS, TM, TV>), dim3(nb), dim3(256), 0, c->stream, L.rp, L.col, val,
+                       rm, n, w, f, x, out);
+  else if (kind == kBlockJacobi)
+    hipLaunchKernelGGL((k_csr_group<NV, kBlockJacobi, 1, TS, TM, TV>), dim3(nb), dim3(256), 0, c->stream, L.rp,
+                       L.col, val, rm, n, w, f, x, out);
+  else
+    hipLaunchKernelGGL((k_csr_group<NV, kJacobi, 1, TS, TM, TV>), dim3(nb), dim3(256), 0, c->stream, L.rp, L.col, val,
+                       rm, n, w, f, x, out);
+}
+
 // t = f - A x (relax = false) or xout = x + W (f - A x) (relax = true) on level l
 template <int NV, typename TM, typename TV>
 static void launch_sweep(bes_ctx *c, Impl *im, Amg *g, int l, bool relax, const TV *f, const TV *x, TV *out) {
@@ -1253,17 +1320,8 @@ static void launch_sweep(bes_ctx *c, Impl *im, Amg *g, int l, bool relax, const 
       launch_ell_sweep<NV>(c, kind, ell<NV, TM>(c, im), n, relax, w, (const float *)nullptr, f, x, out);
     return;
   }
-  const int nb = blocks_for((long)n * 32, 256);
-  const TM *val = (const TM *)L.val;
-  if (!relax)
-    hipLaunchKernelGGL((k_csr_group<NV, kJacobi, 0, TM, TV>), dim3(nb), dim3(256), 0, c->stream, L.rp, L.col, val, n,
-                       w, f, x, out);
-  else if (kind == kBlockJacobi)
-    hipLaunchKernelGGL((k_csr_group<NV, kBlockJacobi, 1, TM, TV>), dim3(nb), dim3(256), 0, c->stream, L.rp, L.col,
-                       val, n, w, f, x, out);
-  else
-    hipLaunchKernelGGL((k_csr_group<NV, kJacobi, 1, TM, TV>), dim3(nb), dim3(256), 0, c->stream, L.rp, L.col, val, n,
-                       w, f, x, out);
+  if (L.hval) launch_csr_sweep<NV>(c, kind, L, relax, (const __half *)L.hval, L.hrm, w, f, x, out);
+  else launch_csr_sweep<NV>(c, kind, L, relax, (const TM *)L.val, (const float *)nullptr, w, f, x, out);
 }
 
 template <int NV, typename TM, typename TV>
@@ -1284,8 +1342,12 @@ static void cycle(bes_ctx *c, Impl *im, Amg *g, int l, const TV *f, TV *x, TV *x
   const Level &C = g->lv[l + 1];
   launch_relax0<NV, TM, TV>(c, g, l, f, xa);
   launch_sweep<NV, TM, TV>(c, im, g, l, false, f, xa, t);
-  hipLaunchKernelGGL((k_restrict<NV, TV>), dim3(blocks_for((long)C.n * 32, 256)), dim3(256), 0, c->stream, C.n,
-                     L.mptr, L.mem, t, (TV *)C.f);
+  if (n <= 12L * C.n)   // small aggregates: 8 lanes each
+    hipLaunchKernelGGL((k_restrict<NV, 8, TV>), dim3(blocks_for((long)C.n * 8, 256)), dim3(256), 0, c->stream, C.n,
+                       L.mptr, L.mem, t, (TV *)C.f);
+  else
+    hipLaunchKernelGGL((k_restrict<NV, 32, TV>), dim3(blocks_for((long)C.n * 32, 256)), dim3(256), 0, c->stream,
+                       C.n, L.mptr, L.mem, t, (TV *)C.f);
   cycle<NV, TM, TV>(c, im, g, l + 1, (const TV *)C.f, (TV *)C.u, (TV *)C.xa, (TV *)C.t);
   hipLaunchKernelGGL((k_prolong<NV, TV>), dim3(nb), dim3(256), 0, c->stream, n, L.agg, (const TV *)C.u, xa);
   launch_sweep<NV, TM, TV>(c, im, g, l, true, f, xa, x);
@@ -1298,7 +1360,7 @@ static double level_bytes(const Amg *g, int nv, size_t es_m, size_t es_v) {
   for (int l = 0; l < g->nlev; l++) {
     const Level &L = g->lv[l];
     const double per = g->prm.relax == kBlockJacobi ? nv * nv : nv;
-    const double em = (l == 0 && g->Ah) ? 2.0 : (double)es_m;
+    const double em = ((l == 0 && g->Ah) || (l > 0 && L.hval)) ? 2.0 : (double)es_m;
     b += 2.0 * L.nnzb * nv * nv * em + 2.0 * L.n * per * em + 8.0 * L.n * nv * es_v;
   }
   if (g->direct) b += (double)g->ndense * g->ndense * 8.0;
@@ -1312,6 +1374,17 @@ using namespace amg;
 int amg_build(bes_ctx *c, Impl *im) {
   if (!im->amg) im->amg = new Amg();
   Amg *g = amg_of(im);
+  // a lagged hierarchy: reused for rebuild - 1 solves after the one that built
+  // it (the cycle then preconditions a nearby matrix; GMRES still applies the
+  // current one)
+  const int type = im->prec == 32 ? 4 : 8;
+  if (g->built_type == type && !g->force && g->reused + 1 < g->prm.rebuild) {
+    g->reused++;
+    return BES_OK;
+  }
+  g->reused = 0;
+  g->force = false;
+  g->builds++;
   const double bytes = (double)c->N * (c->k + 1) * c->nv * c->nv * (im->prec == 32 ? 4.0 : 8.0) * 3.0;
   ProfScope ps(c, kProfFactor, bytes);
   if (im->prec == 32) return c->nd == 2 ? build_t<4, float>(c, im, g) : build_t<5, float>(c, im, g);
@@ -1351,11 +1424,12 @@ using namespace bes;
 extern "C" {
 
 int bes_amg_set_params(bes_ctx *c, double eps_strong, double over_interp, double damping, int coarse_enough,
-                       int relax) {
+                       int relax, int rebuild_interval) {
   Impl *im = c ? (Impl *)c->impl : nullptr;
   if (!im || !(eps_strong > 0 && eps_strong < 1) || !(over_interp >= 1) || !(damping > 0) || coarse_enough < 1 ||
-      relax < 0 || relax > 2) {
-    set_error("call bes_implicit_setup first; 0 < eps_strong < 1, over_interp >= 1, damping > 0, relax 0-2");
+      relax < 0 || relax > 2 || rebuild_interval < 1) {
+    set_error("call bes_implicit_setup first; 0 < eps_strong < 1, over_interp >= 1, damping > 0, relax 0-2, "
+              "rebuild_interval >= 1");
     return BES_ERR_ARG;
   }
   if (!im->amg) im->amg = new Amg();
@@ -1365,15 +1439,18 @@ int bes_amg_set_params(bes_ctx *c, double eps_strong, double over_interp, double
   p.damping = damping;
   p.coarse_enough = coarse_enough;
   p.relax = relax;
+  p.rebuild = rebuild_interval;
+  amg_of(im)->force = true;
   im->factored = false;
   return BES_OK;
 }
 
-int bes_amg_info(bes_ctx *c, int *nlevels, long *rows, long *nnzb, int *direct, int maxlev) {
+int bes_amg_info(bes_ctx *c, int *nlevels, long *rows, long *nnzb, int *direct, long *builds, int maxlev) {
   Impl *im = c ? (Impl *)c->impl : nullptr;
   Amg *g = im ? amg_of(im) : nullptr;
   if (!g || !g->built_type) { set_error("AMG hierarchy not built"); return BES_ERR_ARG; }
   if (nlevels) *nlevels = g->nlev;
+  if (builds) *builds = g->builds;
   for (int l = 0; l < g->nlev && l < maxlev; l++) {
     if (rows) rows[l] = g->lv[l].n;
     if (nnzb) nnzb[l] = g->lv[l].nnzb;

@@ -466,7 +466,7 @@ inline void setupPreconditioner_(bes_ctx *ctx, const Config *config) {
       throw Error(BES_ERR_ARG, "AMG_PRECOND_COARSENER: only AGGREGATION is available");
     if (config->getAmgRelax() < 0)
       throw Error(BES_ERR_ARG, "AMG_PRECOND_SMOOTHER: DAMPED_JACOBI, SPAI0 or BLOCK_JACOBI");
-    check(bes_amg_set_params(ctx, 0.08, 1.5, 0.72, 500, config->getAmgRelax()));
+    check(bes_amg_set_params(ctx, 0.08, 1.5, 0.72, 500, config->getAmgRelax(), 1));
   }
 }
 

@@ -239,9 +239,11 @@ int bes_lp_fallbacks(bes_ctx *ctx);
  * bes_amg_set_params: eps_strong (0.08), over_interp (1.5), damping (0.72),
  *   coarse_enough (500 scalar rows), relax 0 damped Jacobi on the scalar
  *   diagonal (the reference's DAMPED_JACOBI), 1 SPAI0 (its SPAI0), 2 damped
- *   block Jacobi on the nv x nv diagonal blocks (no reference counterpart).
+ *   block Jacobi on the nv x nv diagonal blocks (no reference counterpart);
+ *   rebuild_interval: the hierarchy is rebuilt at every rebuild_interval-th
+ *   solve and reused by the solves between (1: every solve, as AMGCL is).
  * bes_amg_info: levels, block rows and stored blocks per level (up to maxlev),
- *   direct = 1 when the coarsest level is inverted.
+ *   direct = 1 when the coarsest level is inverted, builds so far.
  * bes_amg_level: the aggregate (next-level row, -1 removed) and state (1 root,
  *   2 member, 3 removed) of every row of a level (level 0 in the reference's
  *   cell order), and for levels >= 1 the block CSR of the level's matrix
@@ -249,8 +251,8 @@ int bes_lp_fallbacks(bes_ctx *ctx);
  * bes_amg_apply: one V-cycle z = M r on host vectors [ncell][nv] (reference
  *   cell order), building the hierarchy for the assembled system if needed. ---- */
 int bes_amg_set_params(bes_ctx *ctx, double eps_strong, double over_interp, double damping, int coarse_enough,
-                       int relax);
-int bes_amg_info(bes_ctx *ctx, int *nlevels, long *rows, long *nnzb, int *direct, int maxlev);
+                       int relax, int rebuild_interval);
+int bes_amg_info(bes_ctx *ctx, int *nlevels, long *rows, long *nnzb, int *direct, long *builds, int maxlev);
 int bes_amg_level(bes_ctx *ctx, int level, long *agg, unsigned char *state, long *rowptr, long *col, double *val);
 int bes_amg_apply(bes_ctx *ctx, const double *r, double *z);
 

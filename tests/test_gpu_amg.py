@@ -111,9 +111,9 @@ def test_hierarchy_matches_restatement(dims):
         assert err <= 1e-9
 
 
-@pytest.mark.parametrize("relax,kind", [("spai0", "spai0"), ("block_jacobi", "block")])
+@pytest.mark.parametrize("relax,kind", [("spai0", "spai0"), ("block_jacobi", "block"), ("jacobi", "jacobi")])
 def test_cycle_relaxations_match_restatement(relax, kind):
-    c, A, b = _context((8, 4, 6), relax=relax)
+    c, A, b = _context((16, 8, 12), relax=relax)
     with c:
         rng = np.random.default_rng(8)
         r = rng.standard_normal(b.shape)
@@ -127,7 +127,7 @@ def test_fp32_cycle_close_to_restatement():
     """FP32 solves run the level-0 sweeps from row-scaled FP16 copies of the
     matrix and relaxation blocks; the cycle stays within that rounding of the
     FP64 restatement built from the same aggregates."""
-    c, A, b = _context((8, 4, 6), precision=32, relax="block_jacobi")
+    c, A, b = _context((16, 8, 12), precision=32, relax="block_jacobi")
     with c:
         rng = np.random.default_rng(9)
         r = rng.standard_normal(b.shape)
@@ -162,6 +162,32 @@ def test_fp32_amg_solve():
     xd = spla.spsolve(A.scalar().tocsc(), b)
     assert rr <= 1e-6 * 1.0001
     assert np.abs(x - xd).max() <= 1e-4 * np.abs(xd).max()
+
+
+def test_lagged_hierarchy():
+    """rebuild_interval 3: three assemble-and-solve rounds build one hierarchy,
+    every solve still reaches its tolerance, and new parameters force a build."""
+    md = mg.bump_channel(48, 24, seed=7)
+    finf = orc.freestream_primitive((0.5, 0.0), 2)
+    U = orc.perturbed_state(md.ncell, 2, (0.5, 0.0), seed=2, amp=0.02)
+    with core.Context(md) as c:
+        c.set_freestream(finf)
+        c.set_state(U)
+        c.implicit_setup(32, "amg", rtol=1e-4, maxiter=300)
+        c.amg_set_params(relax="block_jacobi", rebuild_interval=3)
+        its = []
+        for cfl in (10.0, 12.0, 14.0, 16.0):
+            c.implicit_assemble(cfl, repeat_nv=False)
+            it, rr = c.linear_solve()
+            assert rr <= 1e-4 * 1.0001
+            its.append(it)
+            c.implicit_update()
+        assert c.amg_info(with_builds=True)[3] == 2   # solves 1 and 4
+        c.amg_set_params(relax="block_jacobi", rebuild_interval=3)
+        c.implicit_assemble(16.0, repeat_nv=False)
+        c.linear_solve()
+        assert c.amg_info(with_builds=True)[3] == 3
+        print("iterations with a lagged hierarchy", its)
 
 
 def test_amg_implicit_run_converges():
