@@ -76,7 +76,6 @@ struct Level {
   int *agg = nullptr;       // [n] aggregate (= next-level row), -1 removed
   unsigned char *state = nullptr;   // [n] final State of the row
   int *mptr = nullptr, *mem = nullptr;   // members of next-level row I: mem[mptr[I] .. mptr[I+1]), ascending
-  int ncoarse = 0;
   // FP32 solves, coarse levels: row-scaled FP16 copy of val for the sweeps
   __half *hval = nullptr;
   float *hrm = nullptr;
@@ -101,7 +100,6 @@ struct Amg {
   int *flags = nullptr;     // device: [0] undecided count, [1] error bits, [2] pivot failures
   std::vector<int *> scan_tmp;
   void *t0 = nullptr;       // level-0 residual (vector type)
-  size_t vec_es = 0;
   // FP32 solves: row-scaled FP16 copies of the level-0 matrix and relaxation
   // data for the cycle (the hierarchy is built from the full-precision matrix)
   __half *Ah = nullptr, *Wh = nullptr;
@@ -1149,7 +1147,6 @@ static int build_t(bes_ctx *c, Impl *im, Amg *g) {
     g->arena_cap = cap;
   }
   g->arena_need = 0;
-  g->vec_es = es;
   int s;
   const size_t ent0 = (size_t)(K + 1) * N;
   g->lv.resize(1);
@@ -1200,7 +1197,6 @@ static int build_t(bes_ctx *c, Impl *im, Amg *g) {
     if ((s = read_flags(c, g, h))) return s;
     if (h[1] & 1) { set_error("AMG aggregation left a row unassigned"); return BES_ERR_SOLVE; }
     if (nc == 0 || nc >= n) break;   // nothing to coarsen: this level is relaxed as the coarsest
-    L.ncoarse = nc;
     if ((s = members(c, g, l, nc))) return s;
     g->lv.emplace_back();
     Level &Lr = g->lv[l];   // emplace_back may move the vector

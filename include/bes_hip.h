@@ -228,8 +228,9 @@ int bes_lp_fallbacks(bes_ctx *ctx);
 
 /* ---- aggregation AMG preconditioner (precond 4): the AMGCL hierarchy of
  *      CoefficientMatrix.cpp:210-325 with coarsening "aggregation" and
- *      aggr.block_size = nv (:231-256), rebuilt on the GPU for every solve after
- *      an assembly.  Strength eps_strong on the largest |entry| of each block
+ *      aggr.block_size = nv (:231-256), rebuilt on the GPU for the solve after
+ *      an assembly (or every rebuild_interval-th).  FP32 solves sweep row-scaled
+ *      FP16 copies of the level matrices.  Strength eps_strong on the largest |entry| of each block
  *      (AMGCL's plain_aggregates / pointwise_aggregates); aggregates of radius
  *      two around roots that are at least three strong links apart, chosen in
  *      parallel rounds (the reference sweeps the rows in order); coarse
