@@ -139,7 +139,7 @@ def implicit_workload(args):
             c.set_state(U0)
             c.synchronize()
             t = time.perf_counter()
-            hs = [c.implicit_run(50, cmin, cmax, ex, stop_drop=1e-8, repeat_nv=repeat)]
+            hs = [c.implicit_run(50, cmin, cmax, ex, stop_drop=args.drop, repeat_nv=repeat)]
             reached = len(hs[0]) < 50
             n = len(hs[0])
             r0 = hs[0][0, 5]
@@ -156,7 +156,7 @@ def implicit_workload(args):
             st = c.implicit_stats()
             log(f"  {name}: {len(H)} steps, drop {drop:.2e}, {wall:.2f}s, krylov {int(H[:, 11].sum())}, "
                 f"CL {cl:.8f} CD {cd:.8f}")
-            return dict(steps=len(H), wall_s=wall, drop=drop, reached=bool(drop <= 1e-8),
+            return dict(steps=len(H), wall_s=wall, drop=drop, reached=bool(drop <= args.drop),
                         krylov_total=int(H[:, 11].sum()), CL=cl, CD=cd, ilu0_fallbacks=st["lp_fallbacks"],
                         history_density_linf=[float(x) for x in H[::max(1, len(H) // 50), 5]])
 
@@ -182,19 +182,22 @@ def implicit_workload(args):
         out["timed_steps"] = {"cfl_first": float(hist[0, 10]), "cfl_last": float(hist[-1, 10]),
                               "krylov_iterations": int(hist[:, 11].sum())}
         if not args.no_drop:
+            forms = args.forms.split(",")
             plain = drop_run("plain_fp64", 64, "ilu0")
             variants = {"plain_fp64_ilu0": plain}
-            if not args.quick_drop:
+            if "ilu0" in forms and not args.quick_drop:
                 variants["mixed_fp32_ilu0"] = drop_run("mixed_fp32_ilu0", 32, "ilu0")
-            variants["mixed_fp32_learned"] = drop_run("mixed_fp32_learned", 32, "learned")
-            variants["mixed_fp32_amg"] = drop_run("mixed_fp32_amg", 32, "amg")
+            if "learned" in forms:
+                variants["mixed_fp32_learned"] = drop_run("mixed_fp32_learned", 32, "learned")
+            if "amg" in forms:
+                variants["mixed_fp32_amg"] = drop_run("mixed_fp32_amg", 32, "amg")
             for k, v in variants.items():
                 if k != "plain_fp64_ilu0":
                     v["speedup_vs_plain"] = plain["wall_s"] / v["wall_s"]
                     v["dCL_vs_plain"] = abs(v["CL"] - plain["CL"])
                     v["dCD_vs_plain"] = abs(v["CD"] - plain["CD"])
             out["time_to_drop"] = {
-                "target_drop": 1e-8, "monitor": "Linf over cells of the density residual sum_f F*A",
+                "target_drop": args.drop, "monitor": "Linf over cells of the density residual sum_f F*A",
                 "schedule": {"cfl_min": cmin, "cfl_max": cmax, "exponents": list(ex),
                              "jacobian_block_repeat_nv": repeat},
                 **variants,
@@ -459,6 +462,9 @@ def main():
     ap.add_argument("--amg-rebuild", type=int, default=6,
                     help="AMG hierarchy rebuilt at every n-th solve of the mixed AMG run")
     ap.add_argument("--restart", type=int, default=30, help="GMRES restart length of every implicit form")
+    ap.add_argument("--drop", type=float, default=1e-8, help="residual drop of the time-to-drop runs")
+    ap.add_argument("--forms", default="ilu0,learned,amg",
+                    help="mixed forms of the time-to-drop runs besides the plain one (ilu0, learned, amg)")
     ap.add_argument("--rtol", type=float, default=5e-2,
                     help="GMRES relative tolerance of every form (the outer loop needs no tighter)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
