@@ -126,6 +126,7 @@ def implicit_workload(args):
             c.implicit_set_reorth(args.reorth)
             if precond == "learned":
                 c.lp_init(20261202)
+                c.lp_set_rebuild_interval(args.lp_rebuild)
             if precond == "amg":
                 c.amg_set_params(relax="block_jacobi", rebuild_interval=args.amg_rebuild)
 
@@ -202,8 +203,9 @@ def implicit_workload(args):
                              "jacobian_block_repeat_nv": repeat},
                 **variants,
                 "learned_preconditioner": "network with its initial weights (blend 0.953 between block "
-                                          "Jacobi and the first-order Neumann operator); Adam training on "
-                                          "the GPU lowers its loss but not yet its GMRES iterations",
+                                          "Jacobi and the first-order Neumann operator), rebuilt every %d "
+                                          "solves; Adam training on the GPU lowers its loss but not yet its "
+                                          "GMRES iterations" % args.lp_rebuild,
                 "amg_preconditioner": "aggregation AMG (eps_strong 0.08, over_interp 1.5, coarsest <= 500 "
                                       "rows inverted) rebuilt every %d solves, V-cycle with damped (0.72) "
                                       "block Jacobi relaxation; sweeps from row-scaled FP16 copies"
@@ -459,6 +461,8 @@ def main():
                     help="only copy 1 GiB device to device 5 times (calibrates HBM byte counters)")
     ap.add_argument("--timed-form", choices=sorted(TIMED_FORMS), default="plain",
                     help="form of the K timed implicit steps (the headline value is the plain form's)")
+    ap.add_argument("--lp-rebuild", type=int, default=6,
+                    help="learned operator rebuilt at every n-th solve of the mixed learned run")
     ap.add_argument("--amg-rebuild", type=int, default=6,
                     help="AMG hierarchy rebuilt at every n-th solve of the mixed AMG run")
     ap.add_argument("--restart", type=int, default=30, help="GMRES restart length of every implicit form")

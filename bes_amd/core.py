@@ -95,6 +95,7 @@ OPTIONAL_SIGS = {
     "bes_lp_loss_grad": [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p],
     "bes_lp_train": [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_int, C.c_ulonglong, C.c_void_p],
     "bes_lp_fallbacks": [C.c_void_p],
+    "bes_lp_set_rebuild_interval": [C.c_void_p, C.c_int],
     "bes_amg_set_params": [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int],
     "bes_amg_info": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int],
     "bes_amg_level": [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],
@@ -381,6 +382,15 @@ class Context:
 
     def lp_fallbacks(self):
         return lib().bes_lp_fallbacks(self.h)
+
+    def lp_set_rebuild_interval(self, n):
+        _check(lib().bes_lp_set_rebuild_interval(self.h, int(n)))
+
+    def lp_builds(self):
+        f = lib().bes_lp_builds
+        f.restype = C.c_long
+        f.argtypes = [C.c_void_p]
+        return f(self.h)
 
     # ---- aggregation AMG preconditioner ----
     def amg_set_params(self, eps_strong=0.08, over_interp=1.5, damping=0.72, coarse_enough=500, relax="jacobi",

@@ -856,11 +856,24 @@ static int launch_assemble(bes_ctx *c, Impl *im) {
 template <typename TM>
 static int factor(bes_ctx *c, Impl *im) {
   if (im->precond == 4) return amg_build(c, im);   // times itself
+  // a lagged learned operator: kept for lp_rebuild - 1 solves after its build
+  // (GMRES still applies the current matrix)
+  if (im->precond == 3 && im->lp_current && im->lp_since + 1 < im->lp_rebuild) {
+    im->lp_since++;
+    return BES_OK;
+  }
   const int bs = 256, N = c->N, nv = c->nv;
   const size_t mat = (size_t)(c->k + 1) * nv * nv * N;
   const double used = (double)(N + 2 * im->ninternal) * nv * nv * sizeof(TM);   // stored blocks
   ProfScope ps(c, kProfFactor, im->precond == 1 ? 2.0 * N * nv * nv * sizeof(TM) : 3.0 * used);
-  if (im->precond == 3) return lp::lp_build(c, im);
+  if (im->precond == 3) {
+    const int s = lp::lp_build(c, im);
+    if (s) return s;
+    im->lp_since = 0;
+    im->lp_current = true;
+    im->lp_builds++;
+    return BES_OK;
+  }
   if (im->precond == 1) {
     if (c->nd == 2) hipLaunchKernelGGL((k_bj_setup<2, TM>), dim3(grid_for(N, bs)), dim3(bs), 0, c->stream, (const TM *)im->A, (TM *)im->LU, N);
     else hipLaunchKernelGGL((k_bj_setup<3, TM>), dim3(grid_for(N, bs)), dim3(bs), 0, c->stream, (const TM *)im->A, (TM *)im->LU, N);
@@ -1333,6 +1346,7 @@ int bes_linear_solve(bes_ctx *c, int *iters, double *relres) {
     s = solve_once(c, im, &it2, &rr);
     im->precond = 3;
     im->factored = false;
+    im->lp_current = false;   // rebuild the operator for the next solve
     im->lp_fallbacks++;
     it += it2;
     if (s) return s;

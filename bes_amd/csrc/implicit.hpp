@@ -49,6 +49,10 @@ struct Impl {
                            // sets the convergence rate only, never the solution)
   void *lp = nullptr;      // learned-preconditioner network state (learned.hip)
   int lp_fallbacks = 0;    // solves that fell back to ILU(0)
+  int lp_rebuild = 1;      // learned operator rebuilt at every lp_rebuild-th solve after an assembly
+  int lp_since = 0;        // solves since its last build
+  bool lp_current = false; // the stored operator belongs to the present parameters
+  long lp_builds = 0;
   void *amg = nullptr;     // aggregation AMG hierarchy (amg.hip)
   void *V = nullptr;       // Krylov basis (TV) [(m+1)][vstride]
   void *Z = nullptr;       // work vectors (TV) [3][vstride]

@@ -1114,13 +1114,29 @@ int bes_lp_param_count(void) { return NPARAM; }
 int bes_lp_init(bes_ctx *c, unsigned long long seed) {
   Impl *im = impl_get(c);
   if (!im) { set_error("call bes_implicit_setup first"); return BES_ERR_ARG; }
+  im->lp_current = false;
   return lp_alloc(c, im, seed);
+}
+
+int bes_lp_set_rebuild_interval(bes_ctx *c, int n) {
+  Impl *im = impl_get(c);
+  if (!im || n < 1) { set_error("call bes_implicit_setup first / interval >= 1"); return BES_ERR_ARG; }
+  im->lp_rebuild = n;
+  im->lp_current = false;
+  return BES_OK;
+}
+
+long bes_lp_builds(bes_ctx *c) {
+  Impl *im = impl_get(c);
+  return im ? im->lp_builds : 0;
 }
 
 int bes_lp_set_params(bes_ctx *c, const float *params, int n) {
   Impl *im = impl_get(c);
   if (!im || !im->lp || !params || n != NPARAM) { set_error("bes_lp_init first / parameter count"); return BES_ERR_ARG; }
   Lp *L = lp_of(im);
+  im->lp_current = false;
+  im->factored = false;
   BES_HIP(hipMemcpy(L->theta, params, NPARAM * sizeof(float), hipMemcpyHostToDevice));
   BES_HIP(hipMemset(L->m1, 0, NPARAM * sizeof(float)));
   BES_HIP(hipMemset(L->m2, 0, NPARAM * sizeof(float)));
@@ -1224,6 +1240,7 @@ int bes_lp_train(bes_ctx *c, int steps, double lr, int nprobe, int nrich, unsign
     BES_HIP(hipGetLastError());
   }
   im->factored = false;   // the operator changed: rebuild before the next solve
+  im->lp_current = false;
   BES_HIP(hipStreamSynchronize(c->stream));
   return prof_collect(c);
 }

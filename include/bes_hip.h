@@ -225,6 +225,11 @@ int bes_lp_loss_grad(bes_ctx *ctx, const double *probes, int nprobe, int nrich, 
 int bes_lp_train(bes_ctx *ctx, int steps, double lr, int nprobe, int nrich, unsigned long long seed,
                  double *loss_hist);
 int bes_lp_fallbacks(bes_ctx *ctx);
+/* the operator is rebuilt at every n-th solve after an assembly and reused by
+ * the solves between (1: every solve); new parameters or training force a
+ * rebuild.  bes_lp_builds: operator builds so far (no reference counterpart) */
+int bes_lp_set_rebuild_interval(bes_ctx *ctx, int n);
+long bes_lp_builds(bes_ctx *ctx);
 
 /* ---- aggregation AMG preconditioner (precond 4): the AMGCL hierarchy of
  *      CoefficientMatrix.cpp:210-325 with coarsening "aggregation" and

@@ -166,3 +166,27 @@ def test_fallback_to_ilu0_when_learned_stalls():
         assert rr <= 1e-8 * 1.0001
         assert c.lp_fallbacks() == 1
         assert it == it_ilu + it_ilu   # the stalled budget, then the ILU(0) solve
+
+
+def test_lagged_operator_rebuild():
+    """rebuild interval 3: four assemble-and-solve rounds build the operator
+    twice and every solve reaches its tolerance; new parameters force a build."""
+    md = mg.swept_wing(8, 4, 6)
+    finf = orc.freestream_primitive(MVEC, 3)
+    U = orc.perturbed_state(md.ncell, 3, MVEC, seed=6, amp=0.02)
+    with core.Context(md) as c:
+        c.set_freestream(finf)
+        c.set_state(U)
+        c.implicit_setup(32, "learned", restart=30, maxiter=300, rtol=1e-3)
+        c.lp_init(20261202)
+        c.lp_set_rebuild_interval(3)
+        for cfl in (10.0, 11.0, 12.0, 13.0):
+            c.implicit_assemble(cfl)
+            it, rr = c.linear_solve()
+            assert rr <= 1e-3 * 1.0001
+            c.implicit_update()
+        assert c.lp_fallbacks() == 0 and c.lp_builds() == 2
+        c.lp_set_params(c.lp_params())
+        c.implicit_assemble(13.0)
+        c.linear_solve()
+        assert c.lp_builds() == 3
