@@ -100,6 +100,7 @@ OPTIONAL_SIGS = {
     "bes_amg_info": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int],
     "bes_amg_level": [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],
     "bes_amg_apply": [C.c_void_p, C.c_void_p, C.c_void_p],
+    "bes_amg_set_aggregation": [C.c_void_p, C.c_int],
 }
 PROF_KINDS = ["timestep", "gradlim", "residual", "jacobian", "assemble", "spmv", "precond",
               "krylov", "factor", "update", "faceflux", "learned"]
@@ -397,6 +398,10 @@ class Context:
                        rebuild_interval=1):
         _check(lib().bes_amg_set_params(self.h, eps_strong, over_interp, damping, coarse_enough,
                                         self.AMG_RELAX[relax], rebuild_interval))
+
+    def amg_set_aggregation(self, mode):
+        """"parallel" (default) or "row_sweep" (AMGCL's own serial sweep, for checks)."""
+        _check(lib().bes_amg_set_aggregation(self.h, {"parallel": 0, "row_sweep": 1}[mode]))
 
     def amg_info(self, with_builds=False):
         """(rows per level, stored blocks per level, coarsest inverted[, builds so far])."""

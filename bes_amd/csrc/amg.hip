@@ -62,6 +62,7 @@ struct Params {
   int coarse_enough = 500;
   int relax = kJacobi;
   int rebuild = 1;          // hierarchy rebuilt at every rebuild-th solve after an assembly
+  int sweep = 0;            // 1: AMGCL's serial row sweep instead of the parallel aggregation
 };
 
 struct Level {
@@ -381,6 +382,63 @@ __global__ void k_sort_members(int nc, const int *__restrict__ mptr, int *__rest
     while (y >= 0 && m[y] > v) { m[y + 1] = m[y]; y--; }
     m[y + 1] = v;
   }
+}
+
+// AMGCL's row sweep (plain_aggregates.hpp:137-206) in one thread, for checking
+// the hierarchy against the reference's own aggregates: rows in the order
+// order[r] (the reference's cell order on level 0; row order on coarse levels),
+// each unassigned row a root that takes its strong neighbours (even taken
+// ones) and, provisionally, their unassigned strong neighbours; emptied
+// aggregates are dropped.  Serial, so meant for verification-size meshes.
+template <typename Acc>
+__global__ void k_row_sweep(Acc a, int n, const int *__restrict__ order, const unsigned char *__restrict__ strong,
+                            int *__restrict__ id, unsigned char *__restrict__ state, int *__restrict__ used,
+                            int *__restrict__ naggr) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  const int UNDEF = -1, REM = -2;
+  for (int i = 0; i < n; i++) id[i] = state[i] == kRemoved ? REM : UNDEF;
+  int count = 0;
+  for (int r = 0; r < n; r++) {
+    const int i = order ? order[r] : r;
+    if (id[i] != UNDEF) continue;
+    const int cur = count++;
+    id[i] = cur;
+    state[i] = kRoot;
+    const int L = a.len(i);
+    for (int q = 0; q < L; q++) {
+      if (!strong[a.eid(i, q)]) continue;
+      const int c = a.col(i, q);
+      if (id[c] != REM) {
+        id[c] = cur;
+        state[c] = kMember;
+      }
+    }
+    for (int q = 0; q < L; q++) {   // the neighbours just taken: strong, not removed
+      if (!strong[a.eid(i, q)]) continue;
+      const int c = a.col(i, q);
+      if (id[c] == REM) continue;
+      const int Lc = a.len(c);
+      for (int qq = 0; qq < Lc; qq++) {
+        if (!strong[a.eid(c, qq)]) continue;
+        const int cc = a.col(c, qq);
+        if (id[cc] == UNDEF) {
+          id[cc] = cur;
+          state[cc] = kMember;
+        }
+      }
+    }
+  }
+  for (int g = 0; g < count; g++) used[g] = 0;
+  for (int i = 0; i < n; i++)
+    if (id[i] >= 0) used[id[i]] = 1;
+  int m = 0;
+  for (int g = 0; g < count; g++) {
+    const int u = used[g];
+    used[g] = m;
+    m += u;
+  }
+  for (int i = 0; i < n; i++) id[i] = id[i] >= 0 ? used[id[i]] : -1;
+  *naggr = m;
 }
 
 // ---------------------------------------------------------------------------
@@ -999,6 +1057,24 @@ static int aggregate(bes_ctx *c, Amg *g, int l, const Acc &a, int *naggr) {
   const double e2 = (double)((float)g->prm.eps_strong * (float)g->prm.eps_strong);
   hipLaunchKernelGGL((k_pointwise<NV, Acc>), dim3(nb), dim3(bs), 0, c->stream, a, n, g->s, g->dia);
   hipLaunchKernelGGL((k_strong<Acc>), dim3(nb), dim3(bs), 0, c->stream, a, n, g->s, g->dia, e2, g->strong, L.state);
+  BES_HIP(hipGetLastError());
+  if (g->prm.sweep) {
+    int *order = nullptr;
+    if (l == 0) {   // the reference's cell order: step r takes device row iperm[r]
+      std::vector<int> ip(c->iperm.begin(), c->iperm.end());
+      int s = dmalloc(g, (void **)&order, (size_t)n * sizeof(int));
+      if (s) return s;
+      BES_HIP(hipMemcpyAsync(order, ip.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    }
+    hipLaunchKernelGGL((k_row_sweep<Acc>), dim3(1), dim3(64), 0, c->stream, a, n, order, g->strong, L.agg, L.state,
+                       g->ibuf2, g->flags + 3);
+    BES_HIP(hipGetLastError());
+    int h4[4];
+    BES_HIP(hipMemcpyAsync(h4, g->flags, 4 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    BES_HIP(hipStreamSynchronize(c->stream));
+    *naggr = h4[3];
+    return BES_OK;
+  }
   hipLaunchKernelGGL((k_sym<Acc>), dim3(nb), dim3(bs), 0, c->stream, a, n, g->strong, L.state, g->sym);
   BES_HIP(hipGetLastError());
   const unsigned salt = 0x9e3779b9u * (unsigned)(l + 1);
@@ -1436,6 +1512,16 @@ int bes_amg_set_params(bes_ctx *c, double eps_strong, double over_interp, double
   p.coarse_enough = coarse_enough;
   p.relax = relax;
   p.rebuild = rebuild_interval;
+  amg_of(im)->force = true;
+  im->factored = false;
+  return BES_OK;
+}
+
+int bes_amg_set_aggregation(bes_ctx *c, int mode) {
+  Impl *im = c ? (Impl *)c->impl : nullptr;
+  if (!im || mode < 0 || mode > 1) { set_error("call bes_implicit_setup first / mode 0 or 1"); return BES_ERR_ARG; }
+  if (!im->amg) im->amg = new Amg();
+  amg_of(im)->prm.sweep = mode;
   amg_of(im)->force = true;
   im->factored = false;
   return BES_OK;

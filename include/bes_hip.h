@@ -258,6 +258,11 @@ long bes_lp_builds(bes_ctx *ctx);
  *   cell order), building the hierarchy for the assembled system if needed. ---- */
 int bes_amg_set_params(bes_ctx *ctx, double eps_strong, double over_interp, double damping, int coarse_enough,
                        int relax, int rebuild_interval);
+/* aggregation rule: 0 parallel rounds (default), 1 AMGCL's own row sweep in
+ * the reference's cell order (plain_aggregates.hpp:137-206), run serially by
+ * one GPU thread: the reference's aggregates exactly, for verification-size
+ * systems */
+int bes_amg_set_aggregation(bes_ctx *ctx, int mode);
 int bes_amg_info(bes_ctx *ctx, int *nlevels, long *rows, long *nnzb, int *direct, long *builds, int maxlev);
 int bes_amg_level(bes_ctx *ctx, int level, long *agg, unsigned char *state, long *rowptr, long *col, double *val);
 int bes_amg_apply(bes_ctx *ctx, const double *r, double *z);

@@ -111,6 +111,27 @@ def test_hierarchy_matches_restatement(dims):
         assert err <= 1e-9
 
 
+@pytest.mark.parametrize("relax,kind", [("jacobi", "jacobi"), ("block_jacobi", "block")])
+def test_row_sweep_hierarchy_is_amgcl(relax, kind):
+    """With AMGCL's row sweep the GPU builds the reference's hierarchy itself:
+    the aggregates of every level equal the restatement's (in the reference's
+    cell order), and a V-cycle matches the restatement's own hierarchy."""
+    c, A, b = _context((16, 8, 12), relax=relax)
+    with c:
+        c.amg_set_aggregation("row_sweep")
+        z = c.amg_apply(b).ravel()
+        rows, nnzb, direct = c.amg_info()
+        levels, inv = amg.hierarchy(A, kind=kind)
+        assert len(levels) == len(rows) and (inv is not None) == direct
+        for l in range(len(rows) - 1):
+            agg = c.amg_level(l)[0]
+            assert np.array_equal(agg, levels[l].agg), f"level {l} aggregates"
+        zr = amg.vcycle(levels, inv, b)
+        err = np.abs(z - zr).max() / np.abs(zr).max()
+        print("levels", list(rows), "V-cycle rel err %.2e" % err)
+        assert err <= 1e-9
+
+
 @pytest.mark.parametrize("relax,kind", [("spai0", "spai0"), ("block_jacobi", "block"), ("jacobi", "jacobi")])
 def test_cycle_relaxations_match_restatement(relax, kind):
     c, A, b = _context((16, 8, 12), relax=relax)
