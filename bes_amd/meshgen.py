@@ -518,8 +518,12 @@ def swept_wing(ni=96, nj=48, nk=72, seed=20261202, spec: WingSpec = None, jitter
 
 
 def wing_face_mask(md, faces_center, btype):
-    """Slip-wall faces on the wing (not the y = 0 symmetry plane)."""
-    return (btype == SLIP_WALL) & (np.abs(faces_center[:, 1]) > 1e-12)
+    """Slip-wall faces on the wing: not the y = 0 symmetry plane and not the
+    outboard side plane (the largest y of any slip-wall face)."""
+    wall = btype == SLIP_WALL
+    y = faces_center[:, 1]
+    ymax = y[wall].max() if wall.any() else 0.0
+    return wall & (np.abs(y) > 1e-12) & (y < ymax - 1e-9 * max(1.0, abs(ymax)))
 
 
 def force_coefficients(S, center, btype, owner, U, mach, alpha_deg, spec: WingSpec, gamma=1.4):
