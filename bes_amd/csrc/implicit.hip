@@ -743,12 +743,28 @@ static int impl_alloc(bes_ctx *c, Impl *im) {
     color[i] = col;
     ncol = std::max(ncol, col + 1);
   }
+  // Within a colour, rows are grouped by which of their slots are lower (an
+  // earlier colour) and which are boundary faces, Morton order inside a group:
+  // the sweeps and the factorisation branch per slot on that mask, so waves of
+  // one mask load only the blocks they use. Rows of one colour are never
+  // coupled, so this order does not change the factors or any row's arithmetic.
+  std::vector<unsigned char> ckey(N, 0);
+  for (int i = 0; i < N; i++) {
+    const int64_t e = c->perm[i];
+    for (int m = 0; m < k; m++) {
+      const int64_t j = g.cadj[e * k + m];
+      if (j < 0) ckey[i] |= (unsigned char)(1u << (4 + m));
+      else if (color[c->iperm[j]] < color[i]) ckey[i] |= (unsigned char)(1u << m);
+    }
+  }
   std::vector<int> crow;
   crow.reserve(N);
   im->color_start.assign(ncol + 1, 0);
   for (int col = 0; col < ncol; col++) {
     im->color_start[col] = (int)crow.size();
     for (int i = 0; i < N; i++) if (color[i] == col) crow.push_back(i);
+    std::stable_sort(crow.begin() + im->color_start[col], crow.end(),
+                     [&](int a, int b) { return ckey[a] < ckey[b]; });
   }
   im->color_start[ncol] = (int)crow.size();
   im->ncolor = ncol;
