@@ -255,11 +255,13 @@ __global__ void k_perm_rows(const T *__restrict__ src, const int *__restrict__ c
 }
 
 // block ILU(0) of the rows at positions p0 .. p0+nrows-1 (one colour); rows of
-// earlier colours are final.  Elimination order: colour, then position.
+// earlier colours are final.  Elimination order: colour, then cell index (not
+// position, so the factors do not depend on how rows are laid out in a colour).
 template <int ND, typename TM>
 __global__ __launch_bounds__(256) void k_ilu_factor_color(TM *__restrict__ LU, int N, const int *__restrict__ crow,
                                                           const int *__restrict__ cnb, const int *__restrict__ cnp,
-                                                          const unsigned char *__restrict__ clow, int p0, int nrows) {
+                                                          const unsigned char *__restrict__ clow,
+                                                          const int *__restrict__ color, int p0, int nrows) {
   constexpr int NV = ND + 2, K = ND + 1, NB = NV * NV;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nrows) return;
@@ -272,16 +274,23 @@ __global__ __launch_bounds__(256) void k_ilu_factor_color(TM *__restrict__ LU, i
     nb[m] = cnb[(size_t)m * N + p];
     np_[m] = cnp[(size_t)m * N + p];
   }
-  // lower neighbours in elimination order (position order)
-  int ord[K];
+  // lower neighbours in elimination order (colour, then cell index)
+  int ord[K], okey[K];
   int nl = 0;
 #pragma unroll
   for (int m = 0; m < K; m++)
-    if ((lowm >> m) & 1) ord[nl++] = m;
+    if ((lowm >> m) & 1) {
+      okey[nl] = color[nb[m]];
+      ord[nl++] = m;
+    }
   for (int a = 1; a < nl; a++)
     for (int bq = a; bq > 0; bq--) {
       const int x = ord[bq - 1], y = ord[bq];
-      if (np_[x] > np_[y]) { ord[bq - 1] = y; ord[bq] = x; }
+      const int kx = okey[bq - 1], ky = okey[bq];
+      if (kx > ky || (kx == ky && nb[x] > nb[y])) {
+        ord[bq - 1] = y; ord[bq] = x;
+        okey[bq - 1] = ky; okey[bq] = kx;
+      }
     }
   double Aii[NB];
   load_block<TM, NB>(LU, 0, N, p, Aii);
@@ -747,7 +756,9 @@ static int impl_alloc(bes_ctx *c, Impl *im) {
   // earlier colour) and which are boundary faces, Morton order inside a group:
   // the sweeps and the factorisation branch per slot on that mask, so waves of
   // one mask load only the blocks they use. Rows of one colour are never
-  // coupled, so this order does not change the factors or any row's arithmetic.
+  // coupled and the factorisation orders a row's eliminations by colour and
+  // cell index, so this order changes neither the factors nor any row's
+  // arithmetic in the sweeps.
   std::vector<unsigned char> ckey(N, 0);
   for (int i = 0; i < N; i++) {
     const int64_t e = c->perm[i];
@@ -901,10 +912,10 @@ static int factor(bes_ctx *c, Impl *im) {
       if (nr == 0) continue;
       if (c->nd == 2)
         hipLaunchKernelGGL((k_ilu_factor_color<2, TM>), dim3(grid_for(nr, bs)), dim3(bs), 0, c->stream, (TM *)im->LU,
-                           N, im->crow, im->cnb, im->cnp, im->clow, r0, nr);
+                           N, im->crow, im->cnb, im->cnp, im->clow, im->color, r0, nr);
       else
         hipLaunchKernelGGL((k_ilu_factor_color<3, TM>), dim3(grid_for(nr, bs)), dim3(bs), 0, c->stream, (TM *)im->LU,
-                           N, im->crow, im->cnb, im->cnp, im->clow, r0, nr);
+                           N, im->crow, im->cnb, im->cnp, im->clow, im->color, r0, nr);
     }
   }
   BES_HIP(hipGetLastError());
