@@ -352,11 +352,16 @@ __global__ __launch_bounds__(256) void k_ilu_factor_color(TM *__restrict__ LU, i
   store_block<TM, NB>(LU, 0, N, p, Inv);
 }
 
-// forward sweep of one colour (vectors in colour order): y_p = r_p - sum_(k lower) L_pk y_k
-template <int ND, typename TM, typename TV>
+// forward sweep of one colour (vectors in colour order): y_p = r_p - sum_(k lower) L_pk y_k.
+// Rows of colour 0 have no lower neighbours, so y = r there and is never stored:
+// positions below y0end read r. With FUSE (the last colour, which has no upper
+// neighbours) the backward step x_p = inv(U_pp) y_p follows at once, y_p rounded
+// to TV as a store and reload would round it.
+template <int ND, typename TM, typename TV, bool FUSE>
 __global__ __launch_bounds__(256) void k_ilu_lower(const TM *__restrict__ LU, int N, const int *__restrict__ cnp,
                                                    const unsigned char *__restrict__ clow, int p0, int nrows,
-                                                   const TV *__restrict__ r, TV *__restrict__ y) {
+                                                   const TV *__restrict__ r, TV *__restrict__ y, int y0end,
+                                                   TV *__restrict__ x) {
   constexpr int NV = ND + 2, K = ND + 1, NB = NV * NV;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nrows) return;
@@ -369,9 +374,10 @@ __global__ __launch_bounds__(256) void k_ilu_lower(const TM *__restrict__ LU, in
   for (int m = 0; m < K; m++) {
     if (!((lowm >> m) & 1)) continue;
     const int q = cnp[(size_t)m * N + p];
+    const TV *src = q < y0end ? r : y;
     double yk[NV];
 #pragma unroll
-    for (int c = 0; c < NV; c++) yk[c] = (double)y[(size_t)q * NV + c];
+    for (int c = 0; c < NV; c++) yk[c] = (double)src[(size_t)q * NV + c];
     const TM *L = LU + (size_t)(1 + m) * NB * N;
 #pragma unroll
     for (int a = 0; a < NV; a++) {
@@ -381,23 +387,38 @@ __global__ __launch_bounds__(256) void k_ilu_lower(const TM *__restrict__ LU, in
       acc[a] -= s;
     }
   }
+  if constexpr (FUSE) {
 #pragma unroll
-  for (int c = 0; c < NV; c++) y[(size_t)p * NV + c] = (TV)acc[c];
+    for (int c = 0; c < NV; c++) acc[c] = (double)(TV)acc[c];
+#pragma unroll
+    for (int a = 0; a < NV; a++) {
+      double s = 0;
+#pragma unroll
+      for (int c = 0; c < NV; c++) s += (double)LU[((size_t)(a * NV + c)) * N + p] * acc[c];
+      x[(size_t)p * NV + a] = (TV)s;
+    }
+  } else {
+#pragma unroll
+    for (int c = 0; c < NV; c++) y[(size_t)p * NV + c] = (TV)acc[c];
+  }
 }
 
-// backward sweep of one colour (vectors in colour order): x_p = inv(U_pp) (y_p - sum_(j upper) U_pj x_j)
+// backward sweep of one colour (vectors in colour order): x_p = inv(U_pp) (y_p - sum_(j upper) U_pj x_j);
+// y_p is r_p below y0end (colour 0)
 template <int ND, typename TM, typename TV>
 __global__ __launch_bounds__(256) void k_ilu_upper(const TM *__restrict__ LU, int N, const int *__restrict__ cnp,
                                                    const unsigned char *__restrict__ clow, int p0, int nrows,
-                                                   const TV *__restrict__ y, TV *__restrict__ x) {
+                                                   const TV *__restrict__ y, TV *__restrict__ x,
+                                                   const TV *__restrict__ r, int y0end) {
   constexpr int NV = ND + 2, K = ND + 1, NB = NV * NV;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nrows) return;
   const int p = p0 + t;
   const unsigned lowm = clow[p];
+  const TV *own = p < y0end ? r : y;
   double acc[NV];
 #pragma unroll
-  for (int c = 0; c < NV; c++) acc[c] = (double)y[(size_t)p * NV + c];
+  for (int c = 0; c < NV; c++) acc[c] = (double)own[(size_t)p * NV + c];
 #pragma unroll
   for (int m = 0; m < K; m++) {
     const int q = cnp[(size_t)m * N + p];
@@ -442,79 +463,6 @@ __global__ __launch_bounds__(256) void k_bj_apply(const TM *__restrict__ LU, int
 #pragma unroll
     for (int c = 0; c < NV; c++) s += (double)LU[((size_t)(a * NV + c)) * N + i] * ri[c];
     z[(size_t)i * NV + a] = (TV)s;
-  }
-}
-
-// forward sweep of one colour: y_i = r_i - sum_{k lower} L_ik y_k
-template <int ND, typename TM, typename TV>
-__global__ __launch_bounds__(256) void k_ilu_lower(const TM *__restrict__ LU, DevMesh dm,
-                                                   const int *__restrict__ color, const int *__restrict__ rows,
-                                                   int nrows, const TV *__restrict__ r, TV *__restrict__ y) {
-  constexpr int NV = ND + 2, K = ND + 1, NB = NV * NV;
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nrows) return;
-  const int N = dm.N;
-  const int i = rows[t];
-  const int ci = color[i];
-  double acc[NV];
-#pragma unroll
-  for (int c = 0; c < NV; c++) acc[c] = (double)r[(size_t)i * NV + c];
-#pragma unroll
-  for (int m = 0; m < K; m++) {
-    const int k = dm.nbr[(size_t)m * N + i];
-    if (k < 0 || color[k] >= ci) continue;
-    double yk[NV];
-#pragma unroll
-    for (int c = 0; c < NV; c++) yk[c] = (double)y[(size_t)k * NV + c];
-    const TM *L = LU + (size_t)(1 + m) * NB * N;
-#pragma unroll
-    for (int a = 0; a < NV; a++) {
-      double s = 0;
-#pragma unroll
-      for (int c = 0; c < NV; c++) s += (double)L[((size_t)(a * NV + c)) * N + i] * yk[c];
-      acc[a] -= s;
-    }
-  }
-#pragma unroll
-  for (int c = 0; c < NV; c++) y[(size_t)i * NV + c] = (TV)acc[c];
-}
-
-// backward sweep of one colour: x_i = inv(U_ii) (y_i - sum_{j upper} U_ij x_j)
-template <int ND, typename TM, typename TV>
-__global__ __launch_bounds__(256) void k_ilu_upper(const TM *__restrict__ LU, DevMesh dm,
-                                                   const int *__restrict__ color, const int *__restrict__ rows,
-                                                   int nrows, const TV *__restrict__ y, TV *__restrict__ x) {
-  constexpr int NV = ND + 2, K = ND + 1, NB = NV * NV;
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nrows) return;
-  const int N = dm.N;
-  const int i = rows[t];
-  const int ci = color[i];
-  double acc[NV];
-#pragma unroll
-  for (int c = 0; c < NV; c++) acc[c] = (double)y[(size_t)i * NV + c];
-#pragma unroll
-  for (int m = 0; m < K; m++) {
-    const int j = dm.nbr[(size_t)m * N + i];
-    if (j < 0 || color[j] <= ci) continue;
-    double xj[NV];
-#pragma unroll
-    for (int c = 0; c < NV; c++) xj[c] = (double)x[(size_t)j * NV + c];
-    const TM *Uo = LU + (size_t)(1 + m) * NB * N;
-#pragma unroll
-    for (int a = 0; a < NV; a++) {
-      double s = 0;
-#pragma unroll
-      for (int c = 0; c < NV; c++) s += (double)Uo[((size_t)(a * NV + c)) * N + i] * xj[c];
-      acc[a] -= s;
-    }
-  }
-#pragma unroll
-  for (int a = 0; a < NV; a++) {
-    double s = 0;
-#pragma unroll
-    for (int c = 0; c < NV; c++) s += (double)LU[((size_t)(a * NV + c)) * N + i] * acc[c];
-    x[(size_t)i * NV + a] = (TV)s;
   }
 }
 
@@ -929,10 +877,13 @@ static int precond_apply(bes_ctx *c, Impl *im, const TV *r, TV *z, TV *tmp) {
   const int bs = 256, N = c->N;
   const size_t n = (size_t)N * c->nv;
   const double vb = (double)n * sizeof(TV), blocks = (double)(N + 2 * im->ninternal) * c->nv * c->nv;
-  // ILU: every stored block once over both sweeps, r, y (write+read), z,
-  // neighbour ids and colours; Jacobi: diagonal inverse, r, z
+  // ILU: every stored block once over both sweeps, r, z, y written and read for
+  // the colours between the first and the last, neighbour ids and colours;
+  // Jacobi: diagonal inverse, r, z
+  const int nc = im->ncolor;
+  const double ymid = nc > 1 ? (double)(im->color_start[nc - 1] - im->color_start[1]) / std::max(N, 1) : 0.0;
   ProfScope ps(c, kProfPrecond,
-               im->precond == 2 ? blocks * sizeof(TM) + 4.0 * vb + 2.0 * N * (4.0 * c->k + 4.0)
+               im->precond == 2 ? blocks * sizeof(TM) + (2.0 + 2.0 * ymid) * vb + 2.0 * N * (4.0 * c->k + 4.0)
                : im->precond == 3 ? blocks * (im->prec == 32 ? 2.0 : (double)sizeof(TM)) + 2.0 * vb + 4.0 * N * c->k
                : (im->precond == 1 ? (double)N * c->nv * c->nv * sizeof(TM) + 2.0 * vb : 2.0 * vb));
   if (im->precond == 0) {
@@ -947,25 +898,37 @@ static int precond_apply(bes_ctx *c, Impl *im, const TV *r, TV *z, TV *tmp) {
     if (c->nd == 2) hipLaunchKernelGGL((k_bj_apply<2, TM, TV>), dim3(grid_for(N, bs)), dim3(bs), 0, c->stream, (const TM *)im->LU, N, r, z);
     else hipLaunchKernelGGL((k_bj_apply<3, TM, TV>), dim3(grid_for(N, bs)), dim3(bs), 0, c->stream, (const TM *)im->LU, N, r, z);
   } else {
-    for (int col = 0; col < im->ncolor; col++) {
+    // colour 0 needs no forward sweep (y = r); the last colour's backward step
+    // is fused into its forward sweep
+    const int y0end = nc > 0 ? im->color_start[1] : 0;
+    for (int col = 1; col < nc; col++) {
       const int r0 = im->color_start[col], nr = im->color_start[col + 1] - r0;
       if (nr == 0) continue;
-      if (c->nd == 2)
-        hipLaunchKernelGGL((k_ilu_lower<2, TM, TV>), dim3(grid_for(nr, bs)), dim3(bs), 0, c->stream, (const TM *)im->LU,
-                           N, im->cnp, im->clow, r0, nr, r, tmp);
-      else
-        hipLaunchKernelGGL((k_ilu_lower<3, TM, TV>), dim3(grid_for(nr, bs)), dim3(bs), 0, c->stream, (const TM *)im->LU,
-                           N, im->cnp, im->clow, r0, nr, r, tmp);
+      const dim3 g(grid_for(nr, bs));
+      if (col == nc - 1) {
+        if (c->nd == 2)
+          hipLaunchKernelGGL((k_ilu_lower<2, TM, TV, true>), g, dim3(bs), 0, c->stream, (const TM *)im->LU, N, im->cnp,
+                             im->clow, r0, nr, r, tmp, y0end, z);
+        else
+          hipLaunchKernelGGL((k_ilu_lower<3, TM, TV, true>), g, dim3(bs), 0, c->stream, (const TM *)im->LU, N, im->cnp,
+                             im->clow, r0, nr, r, tmp, y0end, z);
+      } else if (c->nd == 2) {
+        hipLaunchKernelGGL((k_ilu_lower<2, TM, TV, false>), g, dim3(bs), 0, c->stream, (const TM *)im->LU, N, im->cnp,
+                           im->clow, r0, nr, r, tmp, y0end, (TV *)nullptr);
+      } else {
+        hipLaunchKernelGGL((k_ilu_lower<3, TM, TV, false>), g, dim3(bs), 0, c->stream, (const TM *)im->LU, N, im->cnp,
+                           im->clow, r0, nr, r, tmp, y0end, (TV *)nullptr);
+      }
     }
-    for (int col = im->ncolor - 1; col >= 0; col--) {
+    for (int col = nc > 1 ? nc - 2 : nc - 1; col >= 0; col--) {
       const int r0 = im->color_start[col], nr = im->color_start[col + 1] - r0;
       if (nr == 0) continue;
       if (c->nd == 2)
         hipLaunchKernelGGL((k_ilu_upper<2, TM, TV>), dim3(grid_for(nr, bs)), dim3(bs), 0, c->stream, (const TM *)im->LU,
-                           N, im->cnp, im->clow, r0, nr, tmp, z);
+                           N, im->cnp, im->clow, r0, nr, tmp, z, r, y0end);
       else
         hipLaunchKernelGGL((k_ilu_upper<3, TM, TV>), dim3(grid_for(nr, bs)), dim3(bs), 0, c->stream, (const TM *)im->LU,
-                           N, im->cnp, im->clow, r0, nr, tmp, z);
+                           N, im->cnp, im->clow, r0, nr, tmp, z, r, y0end);
     }
   }
   BES_HIP(hipGetLastError());
