@@ -597,6 +597,25 @@ __global__ void k_scale_p(const TV *__restrict__ x, double a, TV *__restrict__ y
   for (size_t e = np * W + tid; e < n; e += nt) y[e] = (TV)(a * (double)x[e]);
 }
 
+// w = w / sqrt(n2[0]) when n2[0] > 0 (the norm read on the device, so the
+// launch need not wait for the host); same arithmetic as k_scale_p with
+// a = 1 / sqrt(n2[0])
+template <typename TV>
+__global__ void k_scale_norm(TV *__restrict__ w, const double *__restrict__ n2, size_t n) {
+  const double s2 = n2[0];
+  if (!(s2 > 0)) return;
+  const double a = 1.0 / sqrt(s2);
+  constexpr int W = Pk<TV>::W;
+  const size_t np = n / W, tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x, nt = (size_t)gridDim.x * blockDim.x;
+  for (size_t e = tid; e < np; e += nt) {
+    Pk<TV> v = ldp(w + e * W);
+#pragma unroll
+    for (int t = 0; t < W; t++) v.v[t] = (TV)(a * (double)v.v[t]);
+    stp(w + e * W, v);
+  }
+  for (size_t e = np * W + tid; e < n; e += nt) w[e] = (TV)(a * (double)w[e]);
+}
+
 // y = a*x (scale/copy with conversion)
 template <typename TA, typename TB>
 __global__ void k_scale(const TA *__restrict__ x, double a, TB *__restrict__ y, size_t n) {
@@ -773,6 +792,8 @@ static int impl_alloc(bes_ctx *c, Impl *im) {
   BES_HIP(hipMalloc(&im->Z, 3 * im->vstride * es));
   BES_HIP(hipMalloc((void **)&im->part, 32 * kRedBlocks * sizeof(double)));
   BES_HIP(hipMalloc((void **)&im->dots, 64 * sizeof(double)));
+  BES_HIP(hipHostMalloc((void **)&im->hdots, 64 * sizeof(double), hipHostMallocDefault));
+  BES_HIP(hipEventCreateWithFlags(&im->ev_dots, hipEventDisableTiming));
   BES_HIP(hipMalloc((void **)&im->bits, 16 * sizeof(unsigned long long)));
   return BES_OK;
 }
@@ -786,6 +807,8 @@ static void impl_release(Impl *im) {
                   im->cpos, im->cnb, im->cnp, im->clow};
   for (void *p : ptrs)
     if (p) hipFree(p);
+  if (im->hdots) hipHostFree(im->hdots);
+  if (im->ev_dots) hipEventDestroy(im->ev_dots);
   delete im;
 }
 
@@ -957,10 +980,11 @@ static int spmv(bes_ctx *c, Impl *im, const TV *x, TV *y) {
   return BES_OK;
 }
 
-// dots h[0..cnt) = V[0..cnt)^T w and ||w||^2 (last), copied to host
+// dots h[0..cnt) = V[0..cnt)^T w and ||w||^2 (last), copied to host; with
+// fetch false only queued (chunk q in dots[9q .. 9q+8], ||w||^2 in dots[8])
 template <typename TV>
 static int multidot(bes_ctx *c, Impl *im, const TV *V, size_t stride, int cnt, const TV *w, size_t n, double *h,
-                    double *wnorm2) {
+                    double *wnorm2, bool fetch = true) {
   // chunk q (8 dots + ||w||^2) reduces into dots[9q .. 9q+8]; at most 4 chunks
   const int nchunk = std::max(1, (cnt + 7) / 8);
   if (nchunk > 4) { set_error("too many basis vectors"); return BES_ERR_ARG; }
@@ -974,6 +998,7 @@ static int multidot(bes_ctx *c, Impl *im, const TV *V, size_t stride, int cnt, c
     }
   }
   BES_HIP(hipGetLastError());
+  if (!fetch) return BES_OK;
   double tmp[36];
   BES_HIP(hipMemcpyAsync(tmp, im->dots, sizeof(double) * 9 * nchunk, hipMemcpyDeviceToHost, c->stream));
   BES_HIP(hipStreamSynchronize(c->stream));
@@ -1041,24 +1066,44 @@ static int gmres(bes_ctx *c, Impl *im, int *iters_out, double *relres_out) {
     std::fill(g.begin(), g.end(), 0.0);
     g[0] = rnorm;
     int j = 0;
-    bool done = false;
+    bool done = false, queued = false;
     for (; j < m && total < im->maxiter; j++) {
       TV *vj = V + (size_t)j * vs, *w = V + (size_t)(j + 1) * vs;
       int s;
-      if ((s = precond_apply<TM, TV>(c, im, vj, Z, Z1))) return s;
-      if ((s = spmv<TM, TV>(c, im, Z, w))) return s;
-      // classical Gram-Schmidt with one DGKS re-orthogonalisation
+      if (!queued) {
+        if ((s = precond_apply<TM, TV>(c, im, vj, Z, Z1))) return s;
+        if ((s = spmv<TM, TV>(c, im, Z, w))) return s;
+      }
+      queued = false;
+      // classical Gram-Schmidt with one DGKS re-orthogonalisation. The dots,
+      // the update and (without re-orthogonalisation) the normalisation are
+      // queued back to back and the host waits once, for h and ||w||.
       double w0n2 = 0, wn2 = 0;
-      if ((s = multidot<TV>(c, im, V, vs, j + 1, w, n, hcol.data(), &w0n2))) return s;
+      const bool dev_scale = !(im->reorth_eta > 0);
+      if ((s = multidot<TV>(c, im, V, vs, j + 1, w, n, nullptr, nullptr, false))) return s;
       {
         ProfScope ps(c, kProfKrylov, (double)(j + 3) * n * sizeof(TV));
         hipLaunchKernelGGL((k_orth_update<TV>), dim3(kRedBlocks), dim3(256), 0, c->stream, V, vs, j + 1, im->dots, w,
                            n, im->part);
         hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(256), 0, c->stream, im->part, kRedBlocks, 1, im->dots + 40);
       }
-      BES_HIP(hipMemcpyAsync(&wn2, im->dots + 40, sizeof(double), hipMemcpyDeviceToHost, c->stream));
-      BES_HIP(hipStreamSynchronize(c->stream));
-      if (std::sqrt(wn2) < im->reorth_eta * std::sqrt(w0n2)) {
+      if (dev_scale)
+        hipLaunchKernelGGL((k_scale_norm<TV>), dim3(kRedBlocks), dim3(256), 0, c->stream, w, im->dots + 40, n);
+      BES_HIP(hipMemcpyAsync(im->hdots, im->dots, 41 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+      BES_HIP(hipEventRecord(im->ev_dots, c->stream));
+      // the next step's preconditioner and product need only the normalised w:
+      // queue them before waiting, so the device works while the host reads h
+      // and ||w|| (their results are discarded if this step ends the cycle)
+      if (dev_scale && j + 1 < m && total + 1 < im->maxiter) {
+        if ((s = precond_apply<TM, TV>(c, im, w, Z, Z1))) return s;
+        if ((s = spmv<TM, TV>(c, im, Z, V + (size_t)(j + 2) * vs))) return s;
+        queued = true;
+      }
+      BES_HIP(hipEventSynchronize(im->ev_dots));
+      for (int q = 0; q <= j; q++) hcol[q] = im->hdots[9 * (q / 8) + (q % 8)];
+      w0n2 = im->hdots[8];
+      wn2 = im->hdots[40];
+      if (!dev_scale && std::sqrt(wn2) < im->reorth_eta * std::sqrt(w0n2)) {
         std::vector<double> h2(j + 1);
         if ((s = multidot<TV>(c, im, V, vs, j + 1, w, n, h2.data(), nullptr))) return s;
         for (int q = 0; q <= j; q++) hcol[q] += h2[q];
@@ -1074,7 +1119,7 @@ static int gmres(bes_ctx *c, Impl *im, int *iters_out, double *relres_out) {
       }
       const double hn = std::sqrt(wn2);
       hcol[j + 1] = hn;
-      if (hn > 0)
+      if (hn > 0 && !dev_scale)
         hipLaunchKernelGGL((k_scale_p<TV>), dim3(kRedBlocks), dim3(256), 0, c->stream, w, 1.0 / hn, w, n);
       // Givens rotations
       for (int q = 0; q < j; q++) {

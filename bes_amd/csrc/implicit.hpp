@@ -61,6 +61,8 @@ struct Impl {
   long reorths = 0;        // re-orthogonalisations so far
   double *part = nullptr;  // reduction partials [32][kRedBlocks]
   double *dots = nullptr;  // [64]
+  double *hdots = nullptr; // [64] pinned host copy of dots (Arnoldi step)
+  hipEvent_t ev_dots = nullptr;  // recorded after the copy into hdots
   unsigned long long *bits = nullptr;   // [16] Linf monitors
   int ncolor = 0;
   std::vector<int> color_start;   // host: rows of colour c are crow[color_start[c] .. color_start[c+1])
