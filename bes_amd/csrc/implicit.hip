@@ -261,7 +261,8 @@ template <int ND, typename TM>
 __global__ __launch_bounds__(256) void k_ilu_factor_color(TM *__restrict__ LU, int N, const int *__restrict__ crow,
                                                           const int *__restrict__ cnb, const int *__restrict__ cnp,
                                                           const unsigned char *__restrict__ clow,
-                                                          const int *__restrict__ color, int p0, int nrows) {
+                                                          const int *__restrict__ color, TM *__restrict__ Dr, int p0,
+                                                          int nrows) {
   constexpr int NV = ND + 2, K = ND + 1, NB = NV * NV;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nrows) return;
@@ -299,7 +300,9 @@ __global__ __launch_bounds__(256) void k_ilu_factor_color(TM *__restrict__ LU, i
     const int pk = np_[m];
     double Lik[NB], Uinv[NB], Aik[NB];
     load_block<TM, NB>(LU, 1 + m, N, p, Aik);
-    load_block<TM, NB>(LU, 0, N, pk, Uinv);   // row k is final: slot 0 = inv(U_kk)
+    // row k is final: inv(U_kk), from the row-major copy (one contiguous block)
+#pragma unroll
+    for (int e = 0; e < NB; e++) Uinv[e] = (double)Dr[(size_t)pk * NB + e];
 #pragma unroll
     for (int r = 0; r < NV; r++)
 #pragma unroll
@@ -350,6 +353,8 @@ __global__ __launch_bounds__(256) void k_ilu_factor_color(TM *__restrict__ LU, i
   double Inv[NB];
   block_inverse<NV>(Aii, Inv);
   store_block<TM, NB>(LU, 0, N, p, Inv);
+#pragma unroll
+  for (int e = 0; e < NB; e++) Dr[(size_t)p * NB + e] = (TM)Inv[e];
 }
 
 // forward sweep of one colour (vectors in colour order): y_p = r_p - sum_(k lower) L_pk y_k.
@@ -804,7 +809,7 @@ static void impl_release(Impl *im) {
   void *ptrs[] = {im->fd.fo, im->fd.fn, im->fd.fnrm, im->fd.cf, im->fd.mrev, im->color, im->crow, im->JF,
                   im->b, im->x, im->A, im->LU, im->LM, im->LMh, im->V, im->Z, im->part, im->dots, im->bits, im->Ac,
                   im->bc, im->xc,
-                  im->cpos, im->cnb, im->cnp, im->clow};
+                  im->cpos, im->cnb, im->cnp, im->clow, im->Dr};
   for (void *p : ptrs)
     if (p) hipFree(p);
   if (im->hdots) hipHostFree(im->hdots);
@@ -876,6 +881,7 @@ static int factor(bes_ctx *c, Impl *im) {
     if (c->nd == 2) hipLaunchKernelGGL((k_bj_setup<2, TM>), dim3(grid_for(N, bs)), dim3(bs), 0, c->stream, (const TM *)im->A, (TM *)im->LU, N);
     else hipLaunchKernelGGL((k_bj_setup<3, TM>), dim3(grid_for(N, bs)), dim3(bs), 0, c->stream, (const TM *)im->A, (TM *)im->LU, N);
   } else if (im->precond == 2) {
+    if (!im->Dr) BES_HIP(hipMalloc(&im->Dr, (size_t)N * nv * nv * sizeof(TM)));
     hipLaunchKernelGGL((k_ilu_gather<TM>), dim3(2048), dim3(256), 0, c->stream, (const TM *)im->A, im->crow, N,
                        (c->k + 1) * nv * nv, (TM *)im->Ac, (TM *)im->LU);
     for (int col = 0; col < im->ncolor; col++) {
@@ -883,10 +889,10 @@ static int factor(bes_ctx *c, Impl *im) {
       if (nr == 0) continue;
       if (c->nd == 2)
         hipLaunchKernelGGL((k_ilu_factor_color<2, TM>), dim3(grid_for(nr, bs)), dim3(bs), 0, c->stream, (TM *)im->LU,
-                           N, im->crow, im->cnb, im->cnp, im->clow, im->color, r0, nr);
+                           N, im->crow, im->cnb, im->cnp, im->clow, im->color, (TM *)im->Dr, r0, nr);
       else
         hipLaunchKernelGGL((k_ilu_factor_color<3, TM>), dim3(grid_for(nr, bs)), dim3(bs), 0, c->stream, (TM *)im->LU,
-                           N, im->crow, im->cnb, im->cnp, im->clow, im->color, r0, nr);
+                           N, im->crow, im->cnb, im->cnp, im->clow, im->color, (TM *)im->Dr, r0, nr);
     }
   }
   BES_HIP(hipGetLastError());

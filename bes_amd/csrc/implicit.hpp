@@ -44,6 +44,7 @@ struct Impl {
   double *bc = nullptr;    // right-hand side in colour order
   double *xc = nullptr;    // solution in colour order
   void *LU = nullptr;      // factor / inverse diagonal blocks (TM), [(k+1)*nv*nv][N]
+  void *Dr = nullptr;      // ILU(0): inverse diagonal blocks again, row-major [N][nv*nv] (TM)
   void *LM = nullptr;      // learned operator blocks (TM), [(k+1)*nv*nv][N] (learned.hip)
   __half *LMh = nullptr;   // FP16 copy applied by the FP32 solves (the operator's precision
                            // sets the convergence rate only, never the solution)
