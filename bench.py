@@ -100,18 +100,22 @@ TIMED_FORMS = {"plain": (64, "ilu0"), "mixed-ilu0": (32, "ilu0"), "mixed-learned
                "mixed-amg": (32, "amg")}
 
 
+def next_cfl(cfl, lb_prev, lb_last, cmin, cmax, ex):
+    """Courant number of the step after one run at cfl (SolverImplicit.cpp:530-550)."""
+    e = ex[1] if lb_prev > lb_last else ex[0]
+    return min(cmax, max(cmin, cfl * (lb_prev / lb_last) ** e))
+
+
 def implicit_workload(args):
     from bes_amd import core, meshgen as mg
-    from oracle import oracle as orc
 
     t0 = time.perf_counter()
     md = mg.swept_wing(*args.wing)
     spec = md.wing["spec"]
     alpha = 3.06
     mach = 0.84
-    mvec = (mach * np.cos(np.radians(alpha)), 0.0, mach * np.sin(np.radians(alpha)))
-    finf = orc.freestream_primitive(mvec, 3)
-    U0 = np.tile(orc.conserved_from_primitive(finf, 3), (md.ncell, 1))
+    finf = core.freestream_primitive(core.wing_freestream(mach, alpha), 3)
+    U0 = np.tile(core.conserved_from_primitive(finf, 3), (md.ncell, 1))
     log(f"mesh {md.ncell} tets in {time.perf_counter() - t0:.1f}s")
     cmin, cmax, ex = WING_CFL
     repeat = not args.no_repeat
@@ -159,14 +163,31 @@ def implicit_workload(args):
                 f"CL {cl:.8f} CD {cd:.8f}")
             return dict(steps=len(H), wall_s=wall, drop=drop, reached=bool(drop <= args.drop),
                         krylov_total=int(H[:, 11].sum()), CL=cl, CD=cd, ilu0_fallbacks=st["lp_fallbacks"],
+                        extra_restarts=st.get("extra_restarts", 0),
                         history_density_linf=[float(x) for x in H[::max(1, len(H) // 50), 5]])
 
-        # K timed steps of the plain FP64 solve (or the form --timed-form names)
-        # after W untimed steps of the ramp
+        # The timed steps run at the Courant-number cap: the ramp from CFL 1
+        # runs untimed until a step has used the cap (its first steps take a
+        # few Krylov iterations each and do not represent the solve), then W
+        # more untimed steps, then exactly K timed steps.
         tprec, tprecond = TIMED_FORMS[args.timed_form]
         setup(tprec, tprecond)
         c.set_state(U0)
-        c.implicit_run(args.warmup, cmin, cmax, ex, repeat_nv=repeat)
+        ramp = [c.implicit_run(2, cmin, cmax, ex, repeat_nv=repeat)]
+        nramp = 2
+        while ramp[-1][-1, 10] < cmax and nramp < args.max_ramp:
+            h, _ = c.implicit_continue(5)
+            ramp.append(h)
+            nramp += len(h)
+        if args.warmup > 0:
+            h, _ = c.implicit_continue(args.warmup)
+            ramp.append(h)
+        R = np.concatenate(ramp)
+        log(f"ramp: {nramp} steps to CFL {R[nramp - 1, 10]:.1f}, then {args.warmup} warmup steps")
+        # state, Courant number and monitors at the start of the timed steps
+        # (the CPU baseline continues from exactly this point)
+        U_start = c.get_state()
+        loop = {"cfl": next_cfl(R[-1, 10], R[-2, 0], R[-1, 0], cmin, cmax, ex), "lb_old": R[-1, :5].copy()}
         c.synchronize()
         c.set_profiling(True)
         t = time.perf_counter()
@@ -180,8 +201,12 @@ def implicit_workload(args):
             f"{int(hist[:, 11].sum())} Krylov iterations)")
         value = md.ncell * args.steps / wall
         out["roofline"] = roofline(prof, FORM_KERNEL_NAMES.get(args.timed_form))
-        out["timed_steps"] = {"cfl_first": float(hist[0, 10]), "cfl_last": float(hist[-1, 10]),
-                              "krylov_iterations": int(hist[:, 11].sum())}
+        out["timed_steps"] = {"ramp_steps_to_cap": nramp, "warmup_steps": args.warmup,
+                              "cfl_first": float(hist[0, 10]), "cfl_last": float(hist[-1, 10]),
+                              "cfl_min": float(hist[:, 10].min()),
+                              "krylov_iterations": int(hist[:, 11].sum()),
+                              "krylov_per_step": [int(x) for x in hist[:, 11]],
+                              "density_drop_at_start": float(R[-1, 5] / R[0, 5])}
         if not args.no_drop:
             forms = args.forms.split(",")
             plain = drop_run("plain_fp64", 64, "ilu0")
@@ -202,10 +227,7 @@ def implicit_workload(args):
                 "schedule": {"cfl_min": cmin, "cfl_max": cmax, "exponents": list(ex),
                              "jacobian_block_repeat_nv": repeat},
                 **variants,
-                "learned_preconditioner": "network with its initial weights (blend 0.953 between block "
-                                          "Jacobi and the first-order Neumann operator), rebuilt every %d "
-                                          "solves; Adam training on the GPU lowers its loss but not yet its "
-                                          "GMRES iterations" % args.lp_rebuild,
+                "learned_preconditioner": LEARNED_NOTE % args.lp_rebuild,
                 "amg_preconditioner": "aggregation AMG (eps_strong 0.08, over_interp 1.5, coarsest <= 500 "
                                       "rows inverted) rebuilt every %d solves, V-cycle with damped (0.72) "
                                       "block Jacobi relaxation; sweeps from row-scaled FP16 copies"
@@ -223,15 +245,22 @@ def implicit_workload(args):
                                                   ex),
             "precision": "FP64 throughout" if args.timed_form == "plain" else
                          "FP32 matrix, preconditioner and Krylov solve under the FP64 residual",
-            "timed": f"{args.steps} steps of the {args.timed_form} form after {args.warmup} steps of the ramp"}
+            "timed": f"{args.steps} steps of the {args.timed_form} form at the CFL cap: the ramp ran "
+                     f"{nramp} untimed steps until a step used CFL {cmax:g}, then {args.warmup} untimed steps"}
     res = {"metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps, "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "f64" if tprec == 64 else "f64 residual, f32 solve",
            "data": "synthetic", "config": cfgd}
     res.update(out)
     if not args.no_cpu:
-        res["cpu_baseline"] = cpu_baseline_implicit(args)
+        res["cpu_baseline"] = cpu_baseline_implicit(args, md, finf, U_start, loop, hist, wall / args.steps)
+        if not args.no_config1:
+            res["config1_explicit_bump_40k"] = config1_explicit(args)
     return res
+
+
+LEARNED_NOTE = ("network with its initial weights (blend 0.953 between block Jacobi and the first-order "
+                "Neumann operator), rebuilt every %d solves")
 
 
 def _centers(md, geo):
@@ -246,36 +275,116 @@ def _centers(md, geo):
     return c
 
 
-def cpu_baseline_implicit(args):
-    """The oracle's implicit assembly (reference loops) on a sample, per cell."""
-    from bes_amd import meshgen as mg
+def host_info(threads):
+    """CPU model, core counts and OpenMP settings of the host the baseline ran on."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = None
+    return {"cpu_model": model, "nproc": os.cpu_count(), "affinity_cpus": affinity, "omp_threads": threads,
+            "OMP_PROC_BIND": os.environ.get("OMP_PROC_BIND"), "OMP_PLACES": os.environ.get("OMP_PLACES"),
+            "compiler": "g++ " + ORACLE_FLAGS}
+
+
+ORACLE_FLAGS = "-std=c++17 -O3 -fPIC -fopenmp -ffp-contract=off (oracle/Makefile)"
+
+
+def omp_threads():
+    """Threads of the CPU baseline: the host's CPU share (OMP_NUM_THREADS, else the affinity mask)."""
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        return int(env)
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def cpu_baseline_implicit(args, md, finf, U_start, loop, gpu_hist, gpu_ms_per_step):
+    """The oracle's implicit loop (OpenMP assembly with atomics, multicolour block
+    ILU(0)-GMRES(30) at the same tolerance) continuing from the GPU's state at
+    the start of its timed steps: same mesh, state, Courant number and solver
+    settings.  Steps are timed one at a time until cpu_seconds have passed."""
     from oracle import oracle as orc
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    md = mg.swept_wing(24, 12, 18)
-    mvec = (0.84 * np.cos(np.radians(3.06)), 0.0, 0.84 * np.sin(np.radians(3.06)))
-    finf = orc.freestream_primitive(mvec, 3)
-    U = orc.perturbed_state(md.ncell, 3, mvec, seed=1, amp=0.01)
-    m = orc.Mesh(md)
-    import scipy.sparse as sp
-    import scipy.sparse.linalg as spla
-    nv = 5
+    threads = omp_threads()
     t0 = time.perf_counter()
-    n = 0
-    while time.perf_counter() - t0 < args.cpu_seconds:
-        rp, col, val, rhs, _ = m.implicit_assemble(U, 10.0, finf)
-        # block CSR system, block-Jacobi preconditioned GMRES(30) as the step's solve
-        A = sp.bsr_matrix((val, col, rp), shape=(md.ncell * nv, md.ncell * nv)).tocsr()
-        Dinv = np.linalg.inv(val[rp[:-1]])
-        M = spla.LinearOperator(A.shape, lambda r: np.einsum("nij,nj->ni", Dinv, r.reshape(-1, nv)).ravel())
-        x, info = spla.gmres(A, rhs.ravel(), M=M, restart=30, maxiter=100, rtol=1e-3)
-        U = U + x.reshape(U.shape)
-        n += 1
-    dt = time.perf_counter() - t0
-    return {"value": md.ncell * n / dt, "unit": "cell-updates/s", "cores": 1, "kind": "port",
-            "sample": f"{n} backward-Euler steps on a {md.ncell}-tet wing mesh, {dt:.1f} s: oracle "
-                      f"assembly (reference face loop, Eigen-rule AD Jacobians, serial) + scipy "
-                      f"block-Jacobi GMRES(30) rtol 1e-3 standing in for AMGCL (not buildable here: "
-                      f"needs Boost); one thread (host threads available: {threads})"}
+    m = orc.Mesh(md)
+    s = orc.Implicit(m, restart=args.restart, maxiter=args.krylov_max, rtol=args.rtol, nthreads=threads)
+    cmin, cmax, ex = WING_CFL
+    s.start(cmin, cmax, ex, repeat_nv=not args.no_repeat)
+    s.resume(loop["cfl"], loop["lb_old"], -1.0)
+    U = np.ascontiguousarray(U_start, dtype=np.float64)
+    log(f"cpu baseline: oracle set-up {time.perf_counter() - t0:.1f}s, {threads} threads")
+    hs = []
+    t = time.perf_counter()
+    while True:
+        hs.append(s.steps(U, 1, finf))
+        if time.perf_counter() - t > args.cpu_seconds or len(hs) >= args.steps:
+            break
+    wall = time.perf_counter() - t
+    H = np.concatenate(hs)
+    n = len(H)
+    log(f"cpu baseline: {n} steps in {wall:.1f}s, Krylov {H[:, 11].tolist()}")
+    return {"value": md.ncell * n / wall, "unit": "cell-updates/s", "cores": threads, "kind": "port",
+            "sample": f"{n} backward-Euler step(s) of the same {md.ncell}-tet wing from the GPU's state at the "
+                      f"start of its timed steps (CFL {H[0, 10]:g}), {wall:.1f} s: the oracle's implicit loop "
+                      f"(face-parallel assembly with atomic adds as SolverImplicit.cpp:368, Eigen-rule AD "
+                      f"Jacobians, multicolour block ILU(0)-GMRES({args.restart}) rtol {args.rtol:g} in FP64; the "
+                      f"reference's AMGCL solve cannot be built here) on {threads} OpenMP threads",
+            "ms_per_step": wall * 1e3 / n, "krylov_per_step": [int(x) for x in H[:, 11]],
+            "gpu_same_steps": {"ms_per_step": gpu_ms_per_step,
+                               "krylov_per_step": [int(x) for x in gpu_hist[:n, 11]]},
+            "gpu_over_cpu": (md.ncell * args.steps / (gpu_ms_per_step * args.steps / 1e3)) / (md.ncell * n / wall),
+            "host": host_info(threads)}
+
+
+def config1_explicit(args):
+    """Config 1 in full (2D bump channel, 40k triangles, RK4 at CFL 0.8, 2000
+    iterations from the uniform freestream) on the GPU and with the oracle's
+    OpenMP loop on the host cores: wall time of each, and the final residuals."""
+    from bes_amd import core, meshgen as mg
+    from oracle import oracle as orc
+    md = mg.bump_channel(200, 100)
+    finf = core.freestream_primitive((0.5, 0.0), 2)
+    U0 = np.tile(core.conserved_from_primitive(finf, 2), (md.ncell, 1))
+    niter = args.config1_iters
+    with core.Context(md) as c:
+        c.set_freestream(finf)
+        c.set_state(U0)
+        c.explicit_graph(True)
+        c.explicit_run(2, 0.8)   # graph capture outside the timing
+        c.set_state(U0)
+        c.synchronize()
+        t = time.perf_counter()
+        lg = c.explicit_run(niter, 0.8)
+        c.synchronize()
+        wall_gpu = time.perf_counter() - t
+        Ug = c.get_state()
+    threads = omp_threads()
+    m = orc.Mesh(md)
+    t = time.perf_counter()
+    Uc, lc = m.explicit(U0, niter, 0.8, finf, nthreads=threads)
+    wall_cpu = time.perf_counter() - t
+    log(f"config 1: GPU {wall_gpu:.2f}s, CPU {wall_cpu:.2f}s for {niter} iterations")
+    return {"workload": "explicit-bump-40k", "cells": md.ncell, "iterations": niter,
+            "scheme": "RK4 (4-stage), Roe, Green-Gauss, Venkatakrishnan K=5, local dt, CFL 0.8, FP64",
+            "gpu": {"wall_s": wall_gpu, "cell_updates_per_s": md.ncell * niter / wall_gpu,
+                    "launch": "one HIP graph per pseudo-time step",
+                    "density_linf_drop": float(lg[-1, 0, 0] / lg[0, 0, 0]) if lg.ndim == 3 else
+                    float(lg[-1, 0] / lg[0, 0])},
+            "cpu": {"wall_s": wall_cpu, "cell_updates_per_s": md.ncell * niter / wall_cpu, "cores": threads,
+                    "kind": "port", "density_linf_drop": float(lc[-1, 0] / lc[0, 0])},
+            "gpu_over_cpu": wall_cpu / wall_gpu,
+            "max_abs_state_difference": float(np.abs(Ug - Uc).max())}
 
 
 # --------------------------------------------------------------------------
@@ -284,7 +393,6 @@ def cpu_baseline_implicit(args):
 
 def explicit_workload(args):
     from bes_amd import core, meshgen as mg
-    from oracle import oracle as orc
 
     if args.workload == "explicit-bump-40k":
         md = mg.bump_channel(200, 100)
@@ -300,8 +408,8 @@ def explicit_workload(args):
                f"{md.ncell} tetrahedra", "mach": 0.84, "alpha_deg": 3.06,
                "scheme": "RK4 (4-stage), Roe, Green-Gauss, Venkatakrishnan K=5, local dt, CFL 0.8"}
     nd = md.ndim
-    finf = orc.freestream_primitive(mach, nd)
-    U0 = np.tile(orc.conserved_from_primitive(finf, nd), (md.ncell, 1))
+    finf = core.freestream_primitive(mach, nd)
+    U0 = np.tile(core.conserved_from_primitive(finf, nd), (md.ncell, 1))
     U0 *= 1.0 + 0.01 * np.random.default_rng(3).uniform(-1, 1, U0.shape)
     with core.Context(md) as c:
         c.set_freestream(finf)
@@ -340,15 +448,13 @@ def sweep_conditions(B, seed=20261202):
 def sweep_workload(args):
     """B freestream conditions advancing together on one wing mesh (RK3)."""
     from bes_amd import core, meshgen as mg
-    from oracle import oracle as orc
 
     t0 = time.perf_counter()
     md = mg.swept_wing(*args.sweep_wing)
     B = args.batch
     mach, aoa = sweep_conditions(B)
-    finf = np.stack([orc.freestream_primitive((m * np.cos(np.radians(a)), 0.0, m * np.sin(np.radians(a))), 3)
-                     for m, a in zip(mach, aoa)])
-    U0 = np.stack([np.tile(orc.conserved_from_primitive(f, 3), (md.ncell, 1)) for f in finf])
+    finf = np.stack([core.freestream_primitive(core.wing_freestream(m, a), 3) for m, a in zip(mach, aoa)])
+    U0 = np.stack([np.tile(core.conserved_from_primitive(f, 3), (md.ncell, 1)) for f in finf])
     log(f"mesh {md.ncell} tets x {B} conditions in {time.perf_counter() - t0:.1f}s")
     cfl = 0.8
     with core.Context(md) as c:
@@ -394,7 +500,6 @@ def implicit_sweep_workload(args):
     per launch, so the conditions share geometry, workspace and the AMG arena,
     not launches."""
     from bes_amd import core, meshgen as mg
-    from oracle import oracle as orc
 
     t0 = time.perf_counter()
     md = mg.swept_wing(*args.sweep_wing)
@@ -413,10 +518,9 @@ def implicit_sweep_workload(args):
         t_all = time.perf_counter()
         total_steps = 0
         for q in range(B):
-            mvec = (mach[q] * np.cos(np.radians(aoa[q])), 0.0, mach[q] * np.sin(np.radians(aoa[q])))
-            finf = orc.freestream_primitive(mvec, 3)
+            finf = core.freestream_primitive(core.wing_freestream(mach[q], aoa[q]), 3)
             c.set_freestream(finf)
-            c.set_state(np.tile(orc.conserved_from_primitive(finf, 3), (md.ncell, 1)))
+            c.set_state(np.tile(core.conserved_from_primitive(finf, 3), (md.ncell, 1)))
             c.amg_set_params(relax="block_jacobi", rebuild_interval=args.amg_rebuild)   # fresh hierarchy
             c.synchronize()
             t = time.perf_counter()
@@ -509,8 +613,10 @@ def traffic_from_profiles(kernel):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=None, help="timed steps (implicit 30, explicit 20)")
-    ap.add_argument("--warmup", type=int, default=None, help="untimed steps first (implicit 100, explicit 3)")
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (implicit 20, explicit 20)")
+    ap.add_argument("--warmup", type=int, default=None,
+                    help="untimed steps before the timed ones (implicit: after the ramp has reached the CFL "
+                         "cap; default 5), explicit 3")
     ap.add_argument("--workload", default="implicit-wing-2m",
                     choices=["implicit-wing-2m", "explicit-tetbox", "explicit-bump-40k", "explicit-sweep",
                              "implicit-sweep"])
@@ -540,18 +646,27 @@ def main():
                     help="mixed forms of the time-to-drop runs besides the plain one (ilu0, learned, amg)")
     ap.add_argument("--rtol", type=float, default=5e-2,
                     help="GMRES relative tolerance of every form (the outer loop needs no tighter)")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0,
+                    help="CPU baseline: timed steps continue until this much time has passed (at least one)")
+    ap.add_argument("--max-ramp", type=int, default=300,
+                    help="implicit: untimed ramp steps allowed before the Courant number reaches its cap")
+    ap.add_argument("--config1-iters", type=int, default=2000,
+                    help="iterations of the config-1 explicit GPU-vs-CPU comparison")
+    ap.add_argument("--no-config1", action="store_true", help="skip the config-1 explicit comparison")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-drop", action="store_true")
     args = ap.parse_args()
+    # the CPU baseline's OpenMP placement, read by the runtime when the oracle loads
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
     if args.copy_probe:
         copy_probe()
         return
     implicit = args.workload.startswith("implicit")
     if args.steps is None:
-        args.steps = 30 if implicit else 20
+        args.steps = 20
     if args.warmup is None:
-        args.warmup = 100 if implicit else 3
+        args.warmup = 5 if implicit else 3
     if args.gpus != 1:
         print(json.dumps({"error": "this solver core runs on one GPU; the multi-GPU (partitioned mesh, "
                           "halo exchange) form is designed but not built", "n_gpus": args.gpus}))

@@ -43,6 +43,39 @@ def default_gas(gamma=1.4, R=287.0, harten=0.05, venkat_k=5.0):
     return Gas(gamma, R, harten, venkat_k)
 
 
+def freestream_primitive(mach, ndim, gamma=1.4, R=287.0):
+    """Non-dimensional freestream primitive state [rho, u_d, p, T] of
+    Config::initializePostMesh (Config.cpp:479-488): rho = 1, u = the Mach
+    vector, p = 1/gamma, T = 1/(gamma R), so the sound speed is 1."""
+    p = np.zeros(ndim + 3)
+    p[0] = 1.0
+    p[1:1 + ndim] = np.asarray(mach, dtype=np.float64)[:ndim]
+    p[ndim + 1] = 1.0 / gamma
+    p[ndim + 2] = 1.0 / (gamma * R)
+    return p
+
+
+def conserved_from_primitive(P, ndim, gamma=1.4, R=287.0):
+    """Conserved [rho, rho u_d, rho E] from primitive [rho, u_d, p, T]
+    (GasPerfect.tpp:181-216: E = c_v T + q^2/2); leading axes broadcast."""
+    P = np.asarray(P, dtype=np.float64)
+    cv = R / (gamma - 1.0)
+    U = np.empty(P.shape[:-1] + (ndim + 2,))
+    U[..., 0] = P[..., 0]
+    q2 = np.zeros(P.shape[:-1])
+    for d in range(ndim):
+        U[..., 1 + d] = P[..., 0] * P[..., 1 + d]
+        q2 = q2 + P[..., 1 + d] * P[..., 1 + d]
+    U[..., ndim + 1] = P[..., 0] * (cv * P[..., ndim + 2] + q2 / 2.0)
+    return U
+
+
+def wing_freestream(mach, alpha_deg):
+    """Mach vector of a wing case: x chordwise, z up, angle of attack alpha."""
+    a = np.radians(alpha_deg)
+    return (mach * np.cos(a), 0.0, mach * np.sin(a))
+
+
 _lib = None
 
 _SIGS = {
@@ -81,7 +114,8 @@ OPTIONAL_SIGS = {
     "bes_write_solution": [C.c_void_p, C.c_char_p, C.c_int, C.c_int],
     "bes_write_slip_wall": [C.c_void_p, C.c_char_p, C.c_int],
     "bes_implicit_set_reorth": [C.c_void_p, C.c_double],
-    "bes_implicit_stats": [C.c_void_p, C.c_void_p, C.c_void_p],
+    "bes_implicit_stats": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],
+    "bes_implicit_set_solver": [C.c_void_p, C.c_int],
     "bes_implicit_continue": [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p],
     "bes_set_batch": [C.c_void_p, C.c_int],
     "bes_get_batch": [C.c_void_p],
@@ -298,8 +332,18 @@ class Context:
     def implicit_stats(self):
         r = C.c_long()
         f = C.c_int()
-        _check(lib().bes_implicit_stats(self.h, C.byref(r), C.byref(f)))
-        return {"reorths": r.value, "lp_fallbacks": f.value}
+        e = C.c_long()
+        _check(lib().bes_implicit_stats(self.h, C.byref(r), C.byref(f), C.byref(e)))
+        return {"reorths": r.value, "lp_fallbacks": f.value, "extra_restarts": e.value}
+
+    SOLVER = {"gmres": 0, "fgmres": 0, "bicgstab": 1}
+
+    def implicit_set_solver(self, solver):
+        """Krylov method by the reference's LINEAR_SOLVER names (case-insensitive);
+        FGMRES runs GMRES (the same iterates for a fixed preconditioner); LGMRES
+        is rejected by the library."""
+        code = self.SOLVER.get(str(solver).lower(), -1)
+        _check(lib().bes_implicit_set_solver(self.h, code))
 
     def implicit_set_precond(self, precond):
         _check(lib().bes_implicit_set_precond(self.h, self.PRECOND[precond]))

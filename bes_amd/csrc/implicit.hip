@@ -677,6 +677,46 @@ __global__ void k_fill_u64_i(unsigned long long *p, int n, unsigned long long v)
 }
 
 // ---------------------------------------------------------------------------
+// BiCGStab vector kernels (AMGCL's bicgstab.hpp, right preconditioning); each
+// writes per-block partial sums of the dot products its caller needs next
+// ---------------------------------------------------------------------------
+// p = r + (-beta*omega) v + beta p   (axpbypcz(1, r, -beta*omega, v, beta, p))
+template <typename TV>
+__global__ void k_bicg_p(const TV *__restrict__ r, const TV *__restrict__ v, TV *__restrict__ p, double bo,
+                         double beta, size_t n) {
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
+    p[e] = (TV)((double)r[e] + bo * (double)v[e] + beta * (double)p[e]);
+}
+// dots of a with up to two vectors: (a, u), (a, w); per-block partials [2][grid]
+template <typename TV>
+__global__ __launch_bounds__(256) void k_dot2(const TV *__restrict__ a, const TV *__restrict__ u,
+                                              const TV *__restrict__ w, size_t n, double *__restrict__ part) {
+  double acc[2] = {0, 0};
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+    const double ae = (double)a[e];
+    acc[0] += ae * (double)u[e];
+    if (w) acc[1] += ae * (double)w[e];
+  }
+  block_store_sums(acc, 2, part);
+}
+// x += c T (x FP64); o = i + (-c) y; partials (o, o) and (o, h) (h may be null)
+template <typename TV>
+__global__ __launch_bounds__(256) void k_bicg_step(double *__restrict__ x, const TV *__restrict__ T, double cf,
+                                                   const TV *__restrict__ in, const TV *__restrict__ y,
+                                                   TV *__restrict__ o, const TV *__restrict__ h, size_t n,
+                                                   double *__restrict__ part) {
+  double acc[2] = {0, 0};
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+    x[e] += cf * (double)T[e];
+    const TV oe = (TV)((double)in[e] + (-cf) * (double)y[e]);
+    o[e] = oe;
+    acc[0] += (double)oe * (double)oe;
+    if (h) acc[1] += (double)oe * (double)h[e];
+  }
+  block_store_sums(acc, 2, part);
+}
+
+// ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
 static Impl *impl_of(bes_ctx *c) { return (Impl *)c->impl; }
@@ -793,7 +833,8 @@ static int impl_alloc(bes_ctx *c, Impl *im) {
   BES_HIP(hipMalloc((void **)&im->bc, nvec * sizeof(double)));
   BES_HIP(hipMalloc((void **)&im->xc, nvec * sizeof(double)));
   im->vstride = (nvec + 63) / 64 * 64;   // 16-byte aligned packet loads on every vector
-  BES_HIP(hipMalloc(&im->V, (size_t)(im->restart + 1) * im->vstride * es));
+  // GMRES(m) keeps m + 1 basis vectors; BiCGStab's seven vectors use the same storage
+  BES_HIP(hipMalloc(&im->V, (size_t)std::max(im->restart + 1, 7) * im->vstride * es));
   BES_HIP(hipMalloc(&im->Z, 3 * im->vstride * es));
   BES_HIP(hipMalloc((void **)&im->part, 32 * kRedBlocks * sizeof(double)));
   BES_HIP(hipMalloc((void **)&im->dots, 64 * sizeof(double)));
@@ -1048,9 +1089,13 @@ static int gmres(bes_ctx *c, Impl *im, int *iters_out, double *relres_out) {
   std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), y(m), hcol(m + 1);
   int total = 0;
   double rnorm = bnorm;
-  bool first = true;
+  bool first = true, est_done = false;
   while (total < im->maxiter) {
-    // r = b - A x (x = 0 on the first cycle, as the reference starts every solve)
+    // r = b - A x (x = 0 on the first cycle, as the reference starts every solve).
+    // Every later cycle starts from the true residual of the current x, and the
+    // solve ends only when that is at the target: the Givens estimate of the
+    // cycle before can fall below the true residual (rounding in the basis,
+    // FP32 Krylov vectors), as AMGCL's FGMRES also rechecks (fgmres.hpp:158-163)
     if (first) {
       hipLaunchKernelGGL((k_scale<double, TV>), dim3(kRedBlocks), dim3(256), 0, c->stream, bb, 1.0 / bnorm, V, n);
       rnorm = bnorm;
@@ -1066,9 +1111,11 @@ static int gmres(bes_ctx *c, Impl *im, int *iters_out, double *relres_out) {
       BES_HIP(hipStreamSynchronize(c->stream));
       rnorm = std::sqrt(r2);
       if (rnorm <= target) break;
+      if (est_done) im->extra_restarts++;   // the estimate said done, the true residual did not
       hipLaunchKernelGGL((k_scale_p<TV>), dim3(kRedBlocks), dim3(256), 0, c->stream, Z, 1.0 / rnorm, V, n);
     }
     first = false;
+    est_done = false;
     std::fill(g.begin(), g.end(), 0.0);
     g[0] = rnorm;
     int j = 0;
@@ -1157,7 +1204,7 @@ static int gmres(bes_ctx *c, Impl *im, int *iters_out, double *relres_out) {
     if ((s = precond_apply<TM, TV>(c, im, Z2, Z, Z1))) return s;
     hipLaunchKernelGGL((k_axpy_x<TV>), dim3(kRedBlocks), dim3(256), 0, c->stream, xx, Z, n);
     BES_HIP(hipGetLastError());
-    if (done) break;
+    est_done = done;   // confirmed (or the solve restarted) on the true residual above
   }
   // true residual of the returned solution
   {
@@ -1176,6 +1223,99 @@ static int gmres(bes_ctx *c, Impl *im, int *iters_out, double *relres_out) {
                        im->x);
   *iters_out = total;
   *relres_out = rnorm / bnorm;
+  return BES_OK;
+}
+
+// reduce nq per-block partial sums (written by a kRedBlocks-block kernel) and read them on the host
+static int fetch_sums(bes_ctx *c, Impl *im, int nq, double *out) {
+  hipLaunchKernelGGL(k_reduce_parts, dim3(nq), dim3(256), 0, c->stream, im->part, kRedBlocks, nq, im->dots);
+  BES_HIP(hipGetLastError());
+  BES_HIP(hipMemcpyAsync(im->hdots, im->dots, nq * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  BES_HIP(hipStreamSynchronize(c->stream));
+  for (int q = 0; q < nq; q++) out[q] = im->hdots[q];
+  return BES_OK;
+}
+
+// BiCGStab with right preconditioning, as AMGCL's solver (bicgstab.hpp:
+// operator()): x = 0, r = rh = b; per iteration rho = (r, rh), p, T = M p,
+// v = A T, alpha = rho / (rh, v), x += alpha T, s = r - alpha v; if ||s|| is
+// above the target: T = M s, t = A T, omega = (t, s) / (t, t), x += omega T,
+// r = s - omega t.  Stops when the recursive residual norm is at the target or
+// after maxiter iterations; returns the true relative residual.
+template <typename TM, typename TV>
+static int bicgstab(bes_ctx *c, Impl *im, int *iters_out, double *relres_out) {
+  const size_t n = (size_t)c->N * c->nv, vs = im->vstride;
+  TV *V = (TV *)im->V;
+  TV *r = V, *rh = V + vs, *p = V + 2 * vs, *v = V + 3 * vs, *sv = V + 4 * vs, *t = V + 5 * vs, *T = V + 6 * vs;
+  TV *Z = (TV *)im->Z, *Z1 = Z + vs, *Z2 = Z + 2 * vs;
+  const bool colour = im->precond == 2;
+  const double *bb = im->b;
+  double *xx = colour ? im->xc : im->x;
+  const dim3 G(kRedBlocks), B(256);
+  if (colour) {
+    hipLaunchKernelGGL(k_perm_rows<double>, dim3(1024), B, 0, c->stream, bb, im->crow, c->N, c->nv, 1, im->bc);
+    bb = im->bc;
+  }
+  hipMemsetAsync(xx, 0, n * sizeof(double), c->stream);
+  hipLaunchKernelGGL((k_scale<double, TV>), G, B, 0, c->stream, bb, 1.0, r, n);
+  hipLaunchKernelGGL((k_scale<double, TV>), G, B, 0, c->stream, bb, 1.0, rh, n);
+  double d[2];
+  int st;
+  hipLaunchKernelGGL((k_dot2<TV>), G, B, 0, c->stream, r, r, (const TV *)nullptr, n, im->part);
+  if ((st = fetch_sums(c, im, 1, d))) return st;
+  const double bnorm = std::sqrt(d[0]);
+  *iters_out = 0;
+  *relres_out = 0;
+  if (bnorm == 0) {
+    if (colour) hipMemsetAsync(im->x, 0, n * sizeof(double), c->stream);
+    return BES_OK;
+  }
+  const double target = im->rtol * bnorm;
+  double res = bnorm, rho1 = d[0], rho2 = 0, alpha = 0, omega = 0;   // (r, rh) = ||b||^2 at the start
+  int iter = 0;
+  for (bool first = true; res > target && iter < im->maxiter; ++iter) {
+    rho2 = rho1;
+    if (iter > 0) rho1 = d[1];   // (r, rh) of the r the last iteration ended with
+    if (first) {
+      hipLaunchKernelGGL((k_scale<TV, TV>), G, B, 0, c->stream, r, 1.0, p, n);
+      first = false;
+    } else {
+      if (rho2 == 0 || omega == 0) { set_error("BiCGStab breakdown (zero rho or omega)"); break; }
+      const double beta = (rho1 * alpha) / (rho2 * omega);
+      hipLaunchKernelGGL((k_bicg_p<TV>), G, B, 0, c->stream, r, v, p, -beta * omega, beta, n);
+    }
+    if ((st = precond_apply<TM, TV>(c, im, p, T, Z1))) return st;
+    if ((st = spmv<TM, TV>(c, im, T, v))) return st;
+    hipLaunchKernelGGL((k_dot2<TV>), G, B, 0, c->stream, rh, v, (const TV *)nullptr, n, im->part);
+    if ((st = fetch_sums(c, im, 1, d))) return st;
+    if (d[0] == 0) { set_error("BiCGStab breakdown ((rh, v) = 0)"); break; }
+    alpha = rho1 / d[0];
+    hipLaunchKernelGGL((k_bicg_step<TV>), G, B, 0, c->stream, xx, T, alpha, r, v, sv, (const TV *)nullptr, n,
+                       im->part);
+    if ((st = fetch_sums(c, im, 1, d))) return st;
+    res = std::sqrt(d[0]);
+    if (res > target) {
+      if ((st = precond_apply<TM, TV>(c, im, sv, T, Z1))) return st;
+      if ((st = spmv<TM, TV>(c, im, T, t))) return st;
+      hipLaunchKernelGGL((k_dot2<TV>), G, B, 0, c->stream, t, sv, t, n, im->part);
+      if ((st = fetch_sums(c, im, 2, d))) return st;
+      omega = d[1] > 0 ? d[0] / d[1] : 0.0;
+      if (omega == 0) { set_error("BiCGStab breakdown (omega = 0)"); ++iter; break; }
+      hipLaunchKernelGGL((k_bicg_step<TV>), G, B, 0, c->stream, xx, T, omega, sv, t, r, rh, n, im->part);
+      if ((st = fetch_sums(c, im, 2, d))) return st;
+      res = std::sqrt(d[0]);
+    }
+  }
+  // true residual of the returned solution
+  hipLaunchKernelGGL((k_scale<double, TV>), G, B, 0, c->stream, xx, 1.0, Z2, n);
+  if ((st = spmv<TM, TV>(c, im, Z2, Z1))) return st;
+  hipLaunchKernelGGL((k_resid<TV>), G, B, 0, c->stream, bb, Z1, Z, n, im->part);
+  if ((st = fetch_sums(c, im, 1, d))) return st;
+  if (colour)
+    hipLaunchKernelGGL(k_perm_rows<double>, dim3(1024), B, 0, c->stream, im->xc, im->crow, c->N, c->nv, 0, im->x);
+  BES_HIP(hipGetLastError());
+  *iters_out = iter;
+  *relres_out = std::sqrt(d[0]) / bnorm;
   return BES_OK;
 }
 
@@ -1221,10 +1361,21 @@ int bes_implicit_set_reorth(bes_ctx *c, double eta) {
   return BES_OK;
 }
 
-int bes_implicit_stats(bes_ctx *c, long *reorths, int *lp_fallbacks) {
+int bes_implicit_stats(bes_ctx *c, long *reorths, int *lp_fallbacks, long *extra_restarts) {
   if (!c || !c->impl) { set_error("call bes_implicit_setup first"); return BES_ERR_ARG; }
   if (reorths) *reorths = impl_of(c)->reorths;
   if (lp_fallbacks) *lp_fallbacks = impl_of(c)->lp_fallbacks;
+  if (extra_restarts) *extra_restarts = impl_of(c)->extra_restarts;
+  return BES_OK;
+}
+
+int bes_implicit_set_solver(bes_ctx *c, int solver) {
+  if (!c || !c->impl) { set_error("call bes_implicit_setup first"); return BES_ERR_ARG; }
+  if (solver != BES_SOLVER_GMRES && solver != BES_SOLVER_BICGSTAB) {
+    set_error("LINEAR_SOLVER: only GMRES (FGMRES) and BICGSTAB are available; LGMRES is not");
+    return BES_ERR_ARG;
+  }
+  impl_of(c)->solver = solver;
   return BES_OK;
 }
 
@@ -1364,11 +1515,11 @@ static int solve_once(bes_ctx *c, Impl *im, int *it, double *rr) {
   if (im->prec == 64) {
     if (!im->factored && (s = factor<double>(c, im))) return s;
     im->factored = true;
-    return gmres<double, double>(c, im, it, rr);
+    return im->solver == 1 ? bicgstab<double, double>(c, im, it, rr) : gmres<double, double>(c, im, it, rr);
   }
   if (!im->factored && (s = factor<float>(c, im))) return s;
   im->factored = true;
-  return gmres<float, float>(c, im, it, rr);
+  return im->solver == 1 ? bicgstab<float, float>(c, im, it, rr) : gmres<float, float>(c, im, it, rr);
 }
 
 int bes_linear_solve(bes_ctx *c, int *iters, double *relres) {

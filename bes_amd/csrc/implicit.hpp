@@ -33,6 +33,7 @@ struct Impl {
   int prec = 64;           // 64 or 32
   int precond = 2;         // 0 none, 1 block Jacobi, 2 multicolour ILU(0), 3 learned, 4 AMG
   int restart = 30, maxiter = 1000;
+  int solver = 0;          // 0 GMRES(restart) (also the reference's FGMRES), 1 BiCGStab
   double rtol = 1e-7;
   int repeat_nv = 1;
   FaceDev fd;
@@ -60,6 +61,7 @@ struct Impl {
   size_t vstride = 0;      // N*nv rounded up to 64 elements
   double reorth_eta = 0.7; // DGKS re-orthogonalisation when ||w|| drops below eta of its norm
   long reorths = 0;        // re-orthogonalisations so far
+  long extra_restarts = 0; // GMRES cycles whose Givens estimate met the target but the true residual did not
   double *part = nullptr;  // reduction partials [32][kRedBlocks]
   double *dots = nullptr;  // [64]
   double *hdots = nullptr; // [64] pinned host copy of dots (Arnoldi step)

@@ -54,6 +54,7 @@ inline void check(int s, const bes_ctx *c = nullptr) {
 enum class SolverType { EXPLICIT = 1, IMPLICIT };
 enum class LocalTimeStep { YES = 1, NO };
 enum class AdaptiveCourantNum { YES = 1, NO };
+enum class LinearSolver { FGMRES = 1, GMRES, LGMRES, BICGSTAB };   // Options.hpp:67
 enum VariableType { CONSERVED = 0, PRIMITIVE, COUNT };
 
 // ---------------------------------------------------------------------------
@@ -131,6 +132,7 @@ class Config {
   double getHartenCorrectionMachNum() const { return harten_; }
   double getGasConstant() const { return R_; }
   double getRatioSpecificHeat() const { return gamma_; }
+  LinearSolver getLinearSolverType() const { return linearSolver_; }
   int getLinearSolverNumIter() const { return linMaxIter_; }
   double getLinearSolverTol() const { return linTol_; }
   bes_gas gas() const { return bes_gas{gamma_, R_, harten_, venkat_}; }
@@ -152,6 +154,7 @@ class Config {
     cflMax_ = 0;
     exponent_[0] = exponent_[1] = 1;
     alpha_ = {0.0833, 0.2069, 0.4265, 1.0};
+    linearSolver_ = LinearSolver::FGMRES;
     linMaxIter_ = 1000;
     linTol_ = 1e-7;
     venkat_ = 5.0;
@@ -162,8 +165,7 @@ class Config {
   static double num(const std::string &s) { return std::stod(s); }
   void apply_(const std::string &k, const std::vector<std::string> &a) {
     static const char *ignored[] = {"MESH_FORMAT", "EQUATION",
-                                    "EXPLICIT_SCHEME", "GRADIENT_SCHEME", "LIMITER_SCHEME", "FLUX_SCHEME",
-                                    "LINEAR_SOLVER"};
+                                    "EXPLICIT_SCHEME", "GRADIENT_SCHEME", "LIMITER_SCHEME", "FLUX_SCHEME"};
     for (const char *ig : ignored) if (k == ig) return;   // single scheme each on this path
     if (k == "MESH_FILE") meshFile_ = a[0];
     else if (k == "OUTPUT_FOLDER") outputFolder_ = a[0];
@@ -189,7 +191,12 @@ class Config {
     else if (k == "GAS_CONSTANT") R_ = num(a[0]);
     else if (k == "SPECIFIC_HEAT_RATIO") gamma_ = num(a[0]);
     else if (k == "FREESTREAM_MACH") { mach_.clear(); for (auto &x : a) mach_.push_back(num(x)); }
-    else if (k == "LINEAR_SOLVER_MAX_ITER") linMaxIter_ = (int)num(a[0]);
+    else if (k == "LINEAR_SOLVER") {   // Config.cpp:345-358; other values keep the default
+      if (a[0] == "FGMRES") linearSolver_ = LinearSolver::FGMRES;
+      else if (a[0] == "GMRES") linearSolver_ = LinearSolver::GMRES;
+      else if (a[0] == "LGMRES") linearSolver_ = LinearSolver::LGMRES;
+      else if (a[0] == "BICGSTAB") linearSolver_ = LinearSolver::BICGSTAB;
+    } else if (k == "LINEAR_SOLVER_MAX_ITER") linMaxIter_ = (int)num(a[0]);
     else if (k == "LINEAR_SOLVER_ABS_TOL") linTol_ = num(a[0]);
     // keys of this build
     else if (k == "GEOMETRY_MODE") geomMode_ = a[0] == "EXACT" ? BES_GEOM_EXACT : BES_GEOM_COMPAT;
@@ -211,6 +218,7 @@ class Config {
   unsigned long nRamp_ = 0, nTimeStep_ = 0, nSave_ = 1;
   double rampStart_ = 0.5, cfl_ = 2.0, cflMax_ = 0, exponent_[2] = {1, 1};
   std::vector<double> alpha_, mach_, freestream_, ramp_;
+  LinearSolver linearSolver_ = LinearSolver::FGMRES;
   int linMaxIter_ = 1000;
   double linTol_ = 1e-7, venkat_ = 5.0, harten_ = 0.05, R_ = 287, gamma_ = 1.4;
   int nDim_ = 0, nConserved_ = 0, nPrimitive_ = 0, nCurrentSolution_ = 1;
@@ -460,6 +468,15 @@ typedef FluxJacobianSlipWall FluxJacobianSlipWallThreeDim;
 // configurations (eps_strong 0.08, over_interp 1.5, damping 0.72, coarse_enough
 // 500: CoefficientMatrix.cpp:231-280 with AMGCL's defaults)
 inline void setupPreconditioner_(bes_ctx *ctx, const Config *config) {
+  // Krylov method (CoefficientMatrix.cpp:282-296): FGMRES and GMRES both run
+  // right-preconditioned GMRES, which gives FGMRES's iterates when the
+  // preconditioner is a fixed linear operator (every one here is); LGMRES is
+  // rejected by the library with a message that names the key
+  switch (config->getLinearSolverType()) {
+    case LinearSolver::BICGSTAB: check(bes_implicit_set_solver(ctx, BES_SOLVER_BICGSTAB)); break;
+    case LinearSolver::LGMRES: check(bes_implicit_set_solver(ctx, -1)); break;
+    default: check(bes_implicit_set_solver(ctx, BES_SOLVER_GMRES)); break;
+  }
   if (config->getPreconditioner() == 3) check(bes_lp_init(ctx, 20261202ull));
   if (config->getPreconditioner() == 4) {
     if (!config->getAmgAggregation())

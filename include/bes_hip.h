@@ -157,8 +157,19 @@ int bes_implicit_set_precond(bes_ctx *ctx, int precond);
 /* classical Gram-Schmidt re-orthogonalises (DGKS) when the new Krylov vector
  * keeps less than eta of its norm (default 0.7; 0 never) */
 int bes_implicit_set_reorth(bes_ctx *ctx, double eta);
-/* counters since setup: re-orthogonalisations, learned solves that fell back */
-int bes_implicit_stats(bes_ctx *ctx, long *reorths, int *lp_fallbacks);
+/* counters since setup: re-orthogonalisations, learned solves that fell back,
+ * GMRES cycles whose Givens estimate met the tolerance while the true residual
+ * did not (the solve then restarted from the true residual); any may be NULL */
+int bes_implicit_stats(bes_ctx *ctx, long *reorths, int *lp_fallbacks, long *extra_restarts);
+/* Krylov method of CoefficientMatrix::solve (LINEAR_SOLVER, Config.cpp:345-358;
+ * CoefficientMatrix.cpp:282-296).  BES_SOLVER_GMRES: right-preconditioned
+ * GMRES(restart); it also serves the reference's default FGMRES, which equals
+ * right-preconditioned GMRES whenever the preconditioner is one fixed linear
+ * operator during a solve, as every preconditioner here is.  BES_SOLVER_BICGSTAB:
+ * AMGCL's BiCGStab with right preconditioning (bicgstab.hpp).  The reference's
+ * LGMRES is not available: BES_ERR_ARG. */
+enum { BES_SOLVER_GMRES = 0, BES_SOLVER_BICGSTAB = 1 };
+int bes_implicit_set_solver(bes_ctx *ctx, int solver);
 /* time step, gradient, limiter, face Jacobians and the system A x = b at the
  * current state (SolverImplicit.cpp:307-320 = resetSystemOfEquation_ + dt +
  * gradient + limiter + computeFaceFluxes_).  repeat_nv != 0 reproduces the

@@ -810,3 +810,446 @@ void or_implicit_assemble(const or_mesh *m, const double *U, double cfl, int loc
 }
 
 } // extern "C"
+
+// ---------------------------------------------------------------------------
+// Implicit pseudo-time loop on the CPU (SolverImplicit::solve,
+// SolverImplicit.cpp:552-600), with OpenMP where the reference has it.  Used
+// as the checker's converged solutions and as the timed CPU baseline.
+//   assembly   SolverImplicit.cpp:358-507: face-parallel loop, atomic adds to
+//              the right-hand side and to every block entry (MatrixSparse's
+//              addToVal is atomic), each block added nV times when repeat_nv
+//   solve      the reference calls AMGCL (CoefficientMatrix.cpp:210-325),
+//              which needs Boost and cannot be built here; this restatement
+//              solves with right-preconditioned GMRES(m), classical
+//              Gram-Schmidt, preconditioned by block ILU(0) in a multicolour
+//              order (rows of one colour are independent, so each colour is one
+//              parallel loop), ||b - A x|| <= rtol ||b|| checked on the true
+//              residual before stopping (AMGCL's relative tolerance, Q6)
+//   update     SolverImplicit.cpp:509-528 (U += dU, Linf of b)
+//   CFL        SolverImplicit.cpp:530-550 (adaptive Courant number)
+// ---------------------------------------------------------------------------
+struct or_implicit {
+  int nv = 0, k = 0, restart = 30, maxiter = 300, nthreads = 0;
+  long ne = 0, nnzb = 0;
+  double rtol = 1e-3;
+  std::vector<long> rowptr, bcol, slot;     // BSR: own block, then neighbours in local face order
+  std::vector<int> color;                   // greedy colouring in cell order
+  std::vector<std::vector<long>> rows;      // rows of each colour, ascending
+  std::vector<double> A, LU, Dinv, b, R, x, dt;
+  std::vector<double> V, Zv, w, u;          // Krylov basis [(m+1)][n], work vectors
+  long last_iters = 0;
+};
+
+namespace {
+
+typedef std::vector<double> dvec;
+
+void blk_mul(int nv, const double *A, const double *B, double *C) {   // C = A B
+  for (int r = 0; r < nv; r++)
+    for (int c = 0; c < nv; c++) {
+      double s = 0;
+      for (int p = 0; p < nv; p++) s += A[r * nv + p] * B[p * nv + c];
+      C[r * nv + c] = s;
+    }
+}
+
+void blk_inverse(int nv, const double *Ain, double *Inv) {   // Gauss-Jordan, partial pivoting
+  double A[25];
+  std::copy(Ain, Ain + nv * nv, A);
+  for (int r = 0; r < nv; r++)
+    for (int c = 0; c < nv; c++) Inv[r * nv + c] = r == c ? 1.0 : 0.0;
+  for (int p = 0; p < nv; p++) {
+    int piv = p;
+    for (int r = p + 1; r < nv; r++)
+      if (std::fabs(A[r * nv + p]) > std::fabs(A[piv * nv + p])) piv = r;
+    if (piv != p)
+      for (int c = 0; c < nv; c++) {
+        std::swap(A[p * nv + c], A[piv * nv + c]);
+        std::swap(Inv[p * nv + c], Inv[piv * nv + c]);
+      }
+    const double iv = 1.0 / A[p * nv + p];
+    for (int c = 0; c < nv; c++) { A[p * nv + c] *= iv; Inv[p * nv + c] *= iv; }
+    for (int r = 0; r < nv; r++) {
+      if (r == p) continue;
+      const double f = A[r * nv + p];
+      for (int c = 0; c < nv; c++) { A[r * nv + c] -= f * A[p * nv + c]; Inv[r * nv + c] -= f * Inv[p * nv + c]; }
+    }
+  }
+}
+
+inline void atomic_add(double *p, double v) {
+#pragma omp atomic
+  *p += v;
+}
+
+// y = A x (BSR rows, cell order)
+void imp_spmv(const or_mesh *m, const or_implicit *s, const double *x, double *y) {
+  const int nv = s->nv, nb = nv * nv;
+#pragma omp parallel for
+  for (long e = 0; e < s->ne; e++) {
+    double acc[5] = {0, 0, 0, 0, 0};
+    for (long q = s->rowptr[e]; q < s->rowptr[e + 1]; q++) {
+      const double *B = &s->A[q * nb], *xj = x + s->bcol[q] * nv;
+      for (int r = 0; r < nv; r++)
+        for (int c = 0; c < nv; c++) acc[r] += B[r * nv + c] * xj[c];
+    }
+    for (int r = 0; r < nv; r++) y[e * nv + r] = acc[r];
+  }
+  (void)m;
+}
+
+// block ILU(0): rows eliminated colour by colour; a row's lower neighbours in
+// the order (colour, cell index); LU holds L_ik in lower slots, U_ij in upper
+// slots, Dinv the inverse of each pivot block
+void imp_factor(const or_mesh *m, or_implicit *s) {
+  const int nv = s->nv, nb = nv * nv, k = s->k;
+  s->LU = s->A;
+  for (const auto &rows : s->rows) {
+#pragma omp parallel for schedule(static)
+    for (long q = 0; q < (long)rows.size(); q++) {
+      const long i = rows[q];
+      const int ci = s->color[i];
+      int ord[4], nl = 0;
+      for (int mm = 0; mm < k; mm++) {
+        const long j = m->cadj[i * k + mm];
+        if (j >= 0 && s->color[j] < ci) ord[nl++] = mm;
+      }
+      std::sort(ord, ord + nl, [&](int a, int b) {
+        const long ja = m->cadj[i * k + a], jb = m->cadj[i * k + b];
+        return s->color[ja] != s->color[jb] ? s->color[ja] < s->color[jb] : ja < jb;
+      });
+      double *Aii = &s->LU[s->rowptr[i] * nb];
+      for (int t = 0; t < nl; t++) {
+        const int mm = ord[t];
+        const long kk = m->cadj[i * k + mm];
+        double *Lik = &s->LU[s->slot[i * k + mm] * nb];
+        double tmp[25];
+        blk_mul(nv, Lik, &s->Dinv[kk * nb], tmp);
+        std::copy(tmp, tmp + nb, Lik);
+        for (int mk = 0; mk < k; mk++) {
+          const long j = m->cadj[kk * k + mk];
+          if (j < 0 || s->color[j] < s->color[kk]) continue;   // upper part of row kk only
+          double *dst = nullptr;
+          if (j == i) dst = Aii;
+          else
+            for (int m2 = 0; m2 < k; m2++)
+              if (m->cadj[i * k + m2] == j) dst = &s->LU[s->slot[i * k + m2] * nb];
+          if (!dst) continue;
+          double upd[25];
+          blk_mul(nv, Lik, &s->LU[s->slot[kk * k + mk] * nb], upd);
+          for (int e = 0; e < nb; e++) dst[e] -= upd[e];
+        }
+      }
+      blk_inverse(nv, Aii, &s->Dinv[i * nb]);
+    }
+  }
+}
+
+// z = (LU)^-1 r
+void imp_precond(const or_mesh *m, const or_implicit *s, const double *r, double *z, double *y) {
+  const int nv = s->nv, nb = nv * nv, k = s->k;
+  for (size_t c = 0; c < s->rows.size(); c++) {
+    const auto &rows = s->rows[c];
+#pragma omp parallel for schedule(static)
+    for (long q = 0; q < (long)rows.size(); q++) {
+      const long i = rows[q];
+      double acc[5];
+      for (int a = 0; a < nv; a++) acc[a] = r[i * nv + a];
+      for (int mm = 0; mm < k; mm++) {
+        const long j = m->cadj[i * k + mm];
+        if (j < 0 || s->color[j] >= (int)c) continue;
+        const double *L = &s->LU[s->slot[i * k + mm] * nb];
+        for (int a = 0; a < nv; a++)
+          for (int b = 0; b < nv; b++) acc[a] -= L[a * nv + b] * y[j * nv + b];
+      }
+      for (int a = 0; a < nv; a++) y[i * nv + a] = acc[a];
+    }
+  }
+  for (long c = (long)s->rows.size() - 1; c >= 0; c--) {
+    const auto &rows = s->rows[c];
+#pragma omp parallel for schedule(static)
+    for (long q = 0; q < (long)rows.size(); q++) {
+      const long i = rows[q];
+      double acc[5];
+      for (int a = 0; a < nv; a++) acc[a] = y[i * nv + a];
+      for (int mm = 0; mm < k; mm++) {
+        const long j = m->cadj[i * k + mm];
+        if (j < 0 || s->color[j] <= (int)c) continue;
+        const double *U = &s->LU[s->slot[i * k + mm] * nb];
+        for (int a = 0; a < nv; a++)
+          for (int b = 0; b < nv; b++) acc[a] -= U[a * nv + b] * z[j * nv + b];
+      }
+      const double *D = &s->Dinv[i * nb];
+      for (int a = 0; a < nv; a++) {
+        double t = 0;
+        for (int b = 0; b < nv; b++) t += D[a * nv + b] * acc[b];
+        z[i * nv + a] = t;
+      }
+    }
+  }
+}
+
+double imp_dot(const double *a, const double *b, long n) {
+  double s = 0;
+#pragma omp parallel for reduction(+ : s)
+  for (long e = 0; e < n; e++) s += a[e] * b[e];
+  return s;
+}
+
+// right-preconditioned GMRES(m); x = 0 on entry (SolverImplicit.cpp:349);
+// returns iterations, relres = true ||b - A x|| / ||b||
+long imp_gmres(const or_mesh *m, or_implicit *s, double *relres) {
+  const long n = s->ne * s->nv;
+  const int M = s->restart;
+  dvec &x = s->x;
+  std::fill(x.begin(), x.end(), 0.0);
+  double *V = s->V.data(), *z = s->Zv.data(), *y = s->Zv.data() + n, *w = s->w.data(), *u = s->u.data();
+  const double bnorm = std::sqrt(imp_dot(s->b.data(), s->b.data(), n));
+  *relres = 0;
+  if (bnorm == 0) return 0;
+  const double target = s->rtol * bnorm;
+  std::vector<double> H((size_t)(M + 1) * M), cs(M), sn(M), g(M + 1), yy(M), h(M + 1);
+  long total = 0;
+  bool first = true;
+  double rnorm = bnorm;
+  while (total < s->maxiter) {
+    if (first) {
+#pragma omp parallel for
+      for (long e = 0; e < n; e++) V[e] = s->b[e] / bnorm;
+      first = false;
+    } else {   // true residual of the current x: stop, or restart from it
+      imp_spmv(m, s, x.data(), w);
+#pragma omp parallel for
+      for (long e = 0; e < n; e++) w[e] = s->b[e] - w[e];
+      rnorm = std::sqrt(imp_dot(w, w, n));
+      if (rnorm <= target) break;
+#pragma omp parallel for
+      for (long e = 0; e < n; e++) V[e] = w[e] / rnorm;
+    }
+    std::fill(g.begin(), g.end(), 0.0);
+    g[0] = rnorm;
+    int j = 0;
+    for (; j < M && total < s->maxiter; j++) {
+      double *vj = V + (size_t)j * n, *vn = V + (size_t)(j + 1) * n;
+      imp_precond(m, s, vj, z, y);
+      imp_spmv(m, s, z, vn);
+      for (int q = 0; q <= j; q++) h[q] = imp_dot(V + (size_t)q * n, vn, n);   // classical Gram-Schmidt
+#pragma omp parallel for
+      for (long e = 0; e < n; e++) {
+        double t = vn[e];
+        for (int q = 0; q <= j; q++) t -= h[q] * V[(size_t)q * n + e];
+        vn[e] = t;
+      }
+      const double hn = std::sqrt(imp_dot(vn, vn, n));
+      h[j + 1] = hn;
+      if (hn > 0) {
+#pragma omp parallel for
+        for (long e = 0; e < n; e++) vn[e] /= hn;
+      }
+      for (int q = 0; q < j; q++) {
+        const double t = cs[q] * h[q] + sn[q] * h[q + 1];
+        h[q + 1] = -sn[q] * h[q] + cs[q] * h[q + 1];
+        h[q] = t;
+      }
+      const double den = std::hypot(h[j], h[j + 1]);
+      cs[j] = den > 0 ? h[j] / den : 1.0;
+      sn[j] = den > 0 ? h[j + 1] / den : 0.0;
+      h[j] = den;
+      g[j + 1] = -sn[j] * g[j];
+      g[j] = cs[j] * g[j];
+      for (int q = 0; q <= j; q++) H[(size_t)q * M + j] = h[q];
+      total++;
+      if (std::fabs(g[j + 1]) <= target || hn == 0) { j++; break; }
+    }
+    for (int q = j - 1; q >= 0; q--) {
+      double t = g[q];
+      for (int p = q + 1; p < j; p++) t -= H[(size_t)q * M + p] * yy[p];
+      yy[q] = t / H[(size_t)q * M + q];
+    }
+#pragma omp parallel for
+    for (long e = 0; e < n; e++) {
+      double t = 0;
+      for (int q = 0; q < j; q++) t += yy[q] * V[(size_t)q * n + e];
+      u[e] = t;
+    }
+    imp_precond(m, s, u, z, y);
+#pragma omp parallel for
+    for (long e = 0; e < n; e++) x[e] += z[e];
+  }
+  imp_spmv(m, s, x.data(), w);
+  double r2 = 0;
+#pragma omp parallel for reduction(+ : r2)
+  for (long e = 0; e < n; e++) {
+    const double t = s->b[e] - w[e];
+    r2 += t * t;
+  }
+  *relres = std::sqrt(r2) / bnorm;
+  return total;
+}
+
+// SolverImplicit.cpp:347-507 with the face loop in parallel (atomic adds)
+void imp_assemble(const or_mesh *m, or_implicit *s, const double *U, double cfl, int local,
+                  const double *finf, const or_gas *gas, int repeat_nv) {
+  const int nd = m->nd, nv = m->nv, k = m->k, nb = nv * nv;
+  const long ne = m->ncell;
+  std::vector<double> G(ne * nv * nd), P(ne * nv);
+  or_timestep(m, U, cfl, local, gas, s->dt.data());   // serial, as TimeStepCalculatorLocal
+  or_gradient(m, U, G.data());
+  or_limiter(m, U, G.data(), gas, P.data());
+  std::fill(s->A.begin(), s->A.end(), 0.0);
+  for (long e = 0; e < ne; e++)
+    for (int a = 0; a < nv; a++) s->A[s->rowptr[e] * nb + a * nv + a] = 1.0;
+  std::fill(s->b.begin(), s->b.end(), 0.0);
+  std::fill(s->R.begin(), s->R.end(), 0.0);
+  const int reps = repeat_nv ? nv : 1;
+  const double *dt = s->dt.data();
+  auto blockOf = [&](long row, long col) -> long {
+    if (row == col) return s->rowptr[row];
+    for (int mm = 0; mm < k; mm++) if (m->cadj[row * k + mm] == col) return s->slot[row * k + mm];
+    return -1;
+  };
+  auto add = [&](long row, long col, double mul, const double *J) {
+    double *B = &s->A[blockOf(row, col) * nb];
+    for (int r = 0; r < reps; r++)
+      for (int q = 0; q < nb; q++) atomic_add(&B[q], mul * J[q]);
+  };
+  Gas<double> g(nd, gas);
+#pragma omp parallel for schedule(dynamic, 256)
+  for (long f = 0; f < m->nface; f++) {
+    double L[5], Rv[5], fl[5], JL[25], JR[25], URb[6];
+    const int bt = m->btype[f];
+    faceStates(m, U, G.data(), P.data(), f, L, Rv);
+    anyFlux(g, gas->harten, bt, L, bt == BT_FREESTREAM ? finf : Rv, &m->n[f * nd], fl);
+    const long o = m->owner[f], nbh = m->nbr[f];
+    for (int v = 0; v < nv; v++) {   // Solver.cpp:41-67 (first-order states)
+      L[v] = U[o * nv + v];
+      Rv[v] = nbh >= 0 ? U[nbh * nv + v] : 0.0;
+    }
+    const double *ur = bt == BT_FREESTREAM ? finf : Rv;
+    for (int v = 0; v < nv + 1; v++)
+      URb[v] = (bt == BT_INTERNAL && v < nv) ? ur[v] : (bt == BT_FREESTREAM ? ur[v] : 0.0);
+    jacobianOne(nd, bt, L, URb, &m->n[f * nd], gas, JL, JR);
+    const double A = m->area[f];
+    const double mL = dt[o] / m->vol[o] * A;
+    for (int v = 0; v < nv; v++) {
+      atomic_add(&s->b[o * nv + v], -(mL * fl[v]));
+      atomic_add(&s->R[o * nv + v], fl[v] * A);
+    }
+    add(o, o, mL, JL);
+    if (nbh >= 0) {
+      const double mR = dt[nbh] / m->vol[nbh] * A;
+      for (int v = 0; v < nv; v++) {
+        atomic_add(&s->b[nbh * nv + v], dt[nbh] / m->vol[nbh] * A * fl[v]);
+        atomic_add(&s->R[nbh * nv + v], -(fl[v] * A));
+      }
+      add(o, nbh, mL, JR);
+      add(nbh, o, -mR, JL);
+      add(nbh, nbh, -mR, JR);
+    }
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+or_implicit *or_implicit_create(const or_mesh *m, int restart, int maxiter, double rtol, int nthreads) {
+  or_implicit *s = new or_implicit();
+  const long ne = m->ncell;
+  const int k = m->k, nv = m->nv, nb = nv * nv;
+  s->nv = nv; s->k = k; s->ne = ne; s->restart = restart < 1 ? 1 : restart; s->maxiter = maxiter;
+  s->rtol = rtol; s->nthreads = nthreads;
+  s->rowptr.assign(ne + 1, 0);
+  s->slot.assign(ne * k, -1);
+  for (long e = 0; e < ne; e++) {
+    s->rowptr[e] = (long)s->bcol.size();
+    s->bcol.push_back(e);
+    for (int mm = 0; mm < k; mm++) {
+      const long j = m->cadj[e * k + mm];
+      if (j >= 0) { s->slot[e * k + mm] = (long)s->bcol.size(); s->bcol.push_back(j); }
+    }
+  }
+  s->rowptr[ne] = s->nnzb = (long)s->bcol.size();
+  s->color.assign(ne, -1);
+  int ncol = 0;
+  for (long e = 0; e < ne; e++) {   // greedy colouring in cell order
+    unsigned used = 0;
+    for (int mm = 0; mm < k; mm++) {
+      const long j = m->cadj[e * k + mm];
+      if (j >= 0 && s->color[j] >= 0) used |= 1u << s->color[j];
+    }
+    int c = 0;
+    while (used & (1u << c)) c++;
+    s->color[e] = c;
+    ncol = std::max(ncol, c + 1);
+  }
+  s->rows.assign(ncol, {});
+  for (long e = 0; e < ne; e++) s->rows[s->color[e]].push_back(e);
+  const long n = ne * nv;
+  s->A.assign(s->nnzb * nb, 0.0);
+  s->Dinv.assign(ne * nb, 0.0);
+  s->b.assign(n, 0.0); s->R.assign(n, 0.0); s->x.assign(n, 0.0); s->dt.assign(ne, 0.0);
+  s->V.assign((size_t)(s->restart + 1) * n, 0.0);
+  s->Zv.assign(2 * n, 0.0); s->w.assign(n, 0.0); s->u.assign(n, 0.0);
+  return s;
+}
+
+void or_implicit_destroy(or_implicit *s) { delete s; }
+
+int or_implicit_ncolor(const or_implicit *s) { return (int)s->rows.size(); }
+
+int or_implicit_steps(or_implicit *s, const or_mesh *m, double *U, int nsteps, double *cfl, double cfl_min,
+                      double cfl_max, const double *ex, int local, int repeat_nv, double stop_drop, double r0,
+                      const double *finf, const or_gas *gas, double *lb_old, double *hist) {
+  const int nv = m->nv;
+  const long ne = m->ncell, n = ne * nv;
+#ifdef _OPENMP
+  const int prev = omp_get_max_threads();
+  if (s->nthreads > 0) omp_set_num_threads(s->nthreads);
+#endif
+  int done = 0;
+  for (int it = 0; it < nsteps; it++) {
+    imp_assemble(m, s, U, *cfl, local, finf, gas, repeat_nv);
+    imp_factor(m, s);
+    double rr = 0;
+    const long iters = imp_gmres(m, s, &rr);
+    std::vector<double> lb(nv, 0.0), lr(nv, 0.0);
+    for (long e = 0; e < ne; e++)
+      for (int v = 0; v < nv; v++) {
+        lb[v] = std::max(lb[v], std::fabs(s->b[e * nv + v]));
+        lr[v] = std::max(lr[v], std::fabs(s->R[e * nv + v]));
+      }
+#pragma omp parallel for
+    for (long e = 0; e < n; e++) U[e] += s->x[e];
+    if (hist) {
+      double *h = hist + (size_t)it * (2 * nv + 3);
+      for (int v = 0; v < nv; v++) { h[v] = lb[v]; h[nv + v] = lr[v]; }
+      h[2 * nv] = *cfl;
+      h[2 * nv + 1] = (double)iters;
+      h[2 * nv + 2] = rr;
+    }
+    done = it + 1;
+    if (r0 <= 0) r0 = lr[0];
+    if (cfl_max > cfl_min) {   // SolverImplicit.cpp:530-550
+      const int ei = (lb_old[0] > lb[0]) ? 1 : 0;
+      double c = *cfl * std::pow(lb_old[0] / lb[0], ex[ei]);
+      c = std::max(cfl_min, c);
+      *cfl = std::min(cfl_max, c);
+    }
+    std::copy(lb.begin(), lb.end(), lb_old);
+    if (stop_drop > 0 && lr[0] <= stop_drop * r0) break;
+  }
+#ifdef _OPENMP
+  omp_set_num_threads(prev);
+#endif
+  return done;
+}
+
+/* the system of the last step (BSR as or_implicit_assemble) and its solution */
+void or_implicit_last(const or_implicit *s, double *bval, double *rhs, double *x) {
+  if (bval) std::copy(s->A.begin(), s->A.end(), bval);
+  if (rhs) std::copy(s->b.begin(), s->b.end(), rhs);
+  if (x) std::copy(s->x.begin(), s->x.end(), x);
+}
+
+} // extern "C"

@@ -73,6 +73,30 @@ void or_implicit_assemble(const or_mesh *m, const double *U, double cfl, int loc
                           long *browptr, long *bcol, double *bval, double *rhs,
                           double *dt_out);
 
+/* implicit pseudo-time loop (SolverImplicit.cpp:552-600) with OpenMP: face-
+ * parallel assembly with atomic adds (SolverImplicit.cpp:358-507), then
+ * block ILU(0) in a multicolour order + right-preconditioned GMRES(restart)
+ * to ||b - A x|| <= rtol ||b|| (the reference's AMGCL solve cannot be built
+ * here), U += x, adaptive Courant number (SolverImplicit.cpp:530-550).
+ * nthreads <= 0: the OpenMP default. */
+typedef struct or_implicit or_implicit;
+or_implicit *or_implicit_create(const or_mesh *m, int restart, int maxiter, double rtol, int nthreads);
+void or_implicit_destroy(or_implicit *s);
+int or_implicit_ncolor(const or_implicit *s);
+/* up to nsteps steps in place on U.  *cfl: Courant number of the next step
+ * (updated); lb_old [nv]: Linf of b of the previous step (zeros at the start of
+ * a run: the first step's rule then clamps to cfl_min); stops after the step
+ * whose density Linf of sum F*A is <= stop_drop * r0 (r0 <= 0: this call's
+ * first step).  hist [nsteps][2nv+3]: Linf(b) per var, Linf(sum F*A) per var,
+ * Courant number, Krylov iterations, true relative residual.  Returns steps done. */
+int or_implicit_steps(or_implicit *s, const or_mesh *m, double *U, int nsteps, double *cfl,
+                      double cfl_min, double cfl_max, const double *ex, int local, int repeat_nv,
+                      double stop_drop, double r0, const double *finf, const or_gas *gas,
+                      double *lb_old, double *hist);
+/* the last step's matrix (BSR values as or_implicit_assemble), right-hand side
+ * and linear solution; any may be NULL */
+void or_implicit_last(const or_implicit *s, double *bval, double *rhs, double *x);
+
 #ifdef __cplusplus
 }
 #endif

@@ -62,6 +62,15 @@ def lib():
         _lib.or_implicit_nnzb.restype = C.c_long
         _lib.or_implicit_nnzb.argtypes = [P]
         _lib.or_implicit_assemble.argtypes = [P, P, C.c_double, C.c_int, P, P, C.c_int, P, P, P, P, P]
+        _lib.or_implicit_create.restype = P
+        _lib.or_implicit_create.argtypes = [P, C.c_int, C.c_int, C.c_double, C.c_int]
+        _lib.or_implicit_destroy.argtypes = [P]
+        _lib.or_implicit_ncolor.restype = C.c_int
+        _lib.or_implicit_ncolor.argtypes = [P]
+        _lib.or_implicit_steps.restype = C.c_int
+        _lib.or_implicit_steps.argtypes = [P, P, P, C.c_int, P, C.c_double, C.c_double, P, C.c_int, C.c_int,
+                                           C.c_double, C.c_double, P, P, P, P]
+        _lib.or_implicit_last.argtypes = [P, P, P, P]
     return _lib
 
 
@@ -189,6 +198,65 @@ class Mesh:
         lib().or_implicit_assemble(self.h, _p(U), cfl, int(local), _p(fi), C.byref(gas),
                                    int(repeat_nv), _p(rp), _p(col), _p(val), _p(rhs), _p(dt))
         return rp, col, val, rhs, dt
+
+
+class Implicit:
+    """The implicit pseudo-time loop on the CPU (SolverImplicit.cpp:552-600):
+    OpenMP assembly with atomic adds, multicolour block ILU(0)-GMRES(restart)
+    to the relative tolerance, adaptive Courant number.  Keeps the Courant
+    number and the previous Linf(b) between calls, like the device's
+    bes_implicit_run / bes_implicit_continue."""
+
+    def __init__(self, mesh, restart=30, maxiter=300, rtol=5e-2, nthreads=0):
+        self.mesh = mesh
+        self.h = lib().or_implicit_create(mesh.h, restart, maxiter, rtol, nthreads)
+        self.ncolor = lib().or_implicit_ncolor(self.h)
+        self.cfl = None
+        self.lb_old = np.zeros(mesh.nv)
+        self.r0 = -1.0
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().or_implicit_destroy(self.h)
+            self.h = None
+
+    def start(self, cfl_min, cfl_max, exponents, local=True, repeat_nv=False, stop_drop=0.0):
+        """A new loop: the first step runs at cfl_min (SolverImplicit.cpp:347-356)."""
+        self.cfg = (float(cfl_min), float(cfl_max), np.asarray(exponents, np.float64), int(local),
+                    int(repeat_nv), float(stop_drop))
+        self.cfl = C.c_double(cfl_min)
+        self.lb_old = np.zeros(self.mesh.nv)
+        self.r0 = -1.0
+
+    def resume(self, cfl, lb_old, r0):
+        """Continue a loop at Courant number cfl with the previous step's Linf(b)."""
+        self.cfl = C.c_double(cfl)
+        self.lb_old = np.ascontiguousarray(lb_old, dtype=np.float64).copy()
+        self.r0 = float(r0)
+
+    def steps(self, U, nsteps, finf, gas=None):
+        """Up to nsteps steps in place on U ([ncell][nv] float64, C order); the
+        history rows as bes_implicit_run's."""
+        gas = gas or default_gas()
+        cmin, cmax, ex, local, rep, drop = self.cfg
+        assert U.dtype == np.float64 and U.flags.c_contiguous
+        nv = self.mesh.nv
+        hist = np.zeros((nsteps, 2 * nv + 3))
+        fi = np.ascontiguousarray(finf, dtype=np.float64)
+        n = lib().or_implicit_steps(self.h, self.mesh.h, _p(U), nsteps, C.byref(self.cfl), cmin, cmax, _p(ex),
+                                    local, rep, drop, self.r0, _p(fi), C.byref(gas), _p(self.lb_old), _p(hist))
+        if self.r0 <= 0 and n > 0:
+            self.r0 = float(hist[0, nv])
+        return hist[:n]
+
+    def last_system(self):
+        nv = self.mesh.nv
+        nnzb = lib().or_implicit_nnzb(self.mesh.h)
+        val = np.zeros((nnzb, nv, nv))
+        rhs = np.zeros((self.mesh.ncell, nv))
+        x = np.zeros((self.mesh.ncell, nv))
+        lib().or_implicit_last(self.h, _p(val), _p(rhs), _p(x))
+        return val, rhs, x
 
 
 def flux(ndim, kind, UL, UR, n, gas=None):

@@ -61,8 +61,11 @@ def test_cli_explicit_matches_oracle(tmp_path):
 
 
 @pytest.mark.parametrize("precond", [[], ["PRECONDITIONER = AMG", "AMG_PRECOND_COARSENER = AGGREGATION",
-                                            "AMG_PRECOND_SMOOTHER = BLOCK_JACOBI"]])
+                                            "AMG_PRECOND_SMOOTHER = BLOCK_JACOBI"],
+                                     ["LINEAR_SOLVER = BICGSTAB"], ["LINEAR_SOLVER = GMRES"]])
 def test_cli_implicit_matches_oracle(tmp_path, precond):
+    """Each step's solve against a direct solve of the oracle's system; the
+    reference's Krylov choices (LINEAR_SOLVER, Config.cpp:345-358) included."""
     md = mg.bump_channel(24, 12, seed=10)
     res = _run(tmp_path, md, "IMPLICIT", 5, extra=["LINEAR_SOLVER_ABS_TOL = 1E-12",
                                                    "LINEAR_SOLVER_MAX_ITER = 2000",
@@ -86,7 +89,9 @@ def test_cli_implicit_matches_oracle(tmp_path, precond):
         old = lb
     assert got.shape == (5, 1 + 4 + 2)
     np.testing.assert_allclose(got[:, 1:5], np.array(ref), rtol=1e-5)
-    assert (got[:, 6] <= 1e-12 * 1.0001).all()
+    # column 6: the true relative residual of each solve (BiCGStab stops on its
+    # recursive residual, as AMGCL's does, so its true residual may sit a little above)
+    assert (got[:, 6] <= (1e-11 if "BICGSTAB" in " ".join(precond) else 1e-12 * 1.0001)).all()
     # SAVE_INTERVAL 2: after steps 1, 3 and 5 (0-based 0, 2, 4), CSV and slip wall
     names = sorted(p.name for p in tmp_path.glob("s*_*.csv"))
     assert names == ["slip_wall_1.csv", "slip_wall_3.csv", "slip_wall_5.csv",
@@ -101,6 +106,12 @@ def test_cli_rejects_unavailable_amg_smoother(tmp_path):
     md = mg.tri_patch(3, 3)
     res = _run(tmp_path, md, "IMPLICIT", 1, extra=["PRECONDITIONER = AMG", "AMG_PRECOND_SMOOTHER = GAUSS_SEIDEL"])
     assert res.returncode == 1 and "AMG_PRECOND_SMOOTHER" in res.stderr + res.stdout
+
+
+def test_cli_rejects_lgmres(tmp_path):
+    md = mg.tri_patch(3, 3)
+    res = _run(tmp_path, md, "IMPLICIT", 1, extra=["LINEAR_SOLVER = LGMRES"])
+    assert res.returncode == 1 and "LINEAR_SOLVER" in res.stderr + res.stdout
 
 
 def test_cli_rejects_unknown_key(tmp_path):
