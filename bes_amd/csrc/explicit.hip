@@ -1058,6 +1058,34 @@ int bes_explicit_run(bes_ctx *c, int niter, int nstage, const double *alpha, dou
   return check_err(c);
 }
 
+// Staging of bes_flux_batch_noalloc: a file-scope device array, one copy per
+// GPU for the life of the process (kNoAllocMax faces: UL [n][5], UR [n][6],
+// normals [n][3], F [n][5]; 2.5 MB).  Host code never passes the array's name
+// to a kernel or a copy: it takes the device address from hipGetSymbolAddress
+// once per device, with that device current, and uses that address.
+constexpr size_t kNoAllocMax = BES_NOALLOC_MAX_FACES;
+static __device__ __align__(256) double scratch[kNoAllocMax * (5 + 6 + 3 + 5)];
+
+__global__ void k_zero_f64(double *p, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0;
+}
+
+static int noalloc_scratch(double **out) {
+  constexpr int kMaxDevices = 64;
+  static double *ptr[kMaxDevices] = {nullptr};
+  int dev = 0;
+  BES_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) { set_error("device index out of range"); return BES_ERR_ARG; }
+  if (!ptr[dev]) {
+    void *p = nullptr;
+    BES_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    ptr[dev] = (double *)p;
+  }
+  *out = ptr[dev];
+  return BES_OK;
+}
+
 static int batch_common(int ndim, int kind, long n, const double *UL, const double *UR,
                         const double *nrm, const bes_gas *gas, double *out1, double *out2, bool jac) {
   if ((ndim != 2 && ndim != 3) || n < 0 || (kind < 1 || kind > 3) || (n > 0 && (!UL || !nrm || !out1)) ||
@@ -1102,6 +1130,43 @@ static int batch_common(int ndim, int kind, long n, const double *UL, const doub
 int bes_flux_batch(int ndim, int kind, long n, const double *UL, const double *UR, const double *normals,
                    const bes_gas *gas, double *F) {
   return batch_common(ndim, kind, n, UL, UR, normals, gas, F, nullptr, false);
+}
+
+int bes_flux_batch_noalloc(int ndim, int kind, long n, const double *UL, const double *UR, const double *normals,
+                           const bes_gas *gas, double *F) {
+  if ((ndim != 2 && ndim != 3) || n < 0 || (kind < 1 || kind > 3) || (n > 0 && (!UL || !normals || !F)) ||
+      (kind != kSlipWall && n > 0 && !UR)) {
+    set_error("invalid batch argument");
+    return BES_ERR_ARG;
+  }
+  if (n > BES_NOALLOC_MAX_FACES) {
+    set_error("bes_flux_batch_noalloc: n = " + std::to_string(n) + " exceeds the staging limit of " +
+              std::to_string(BES_NOALLOC_MAX_FACES) + " faces");
+    return BES_ERR_ARG;
+  }
+  if (n == 0) return BES_OK;
+  double *base = nullptr;
+  int s = noalloc_scratch(&base);
+  if (s) return s;
+  const int nv = ndim + 2;
+  const size_t nn = (size_t)n;
+  // staging regions, each at the offset the largest batch would give it
+  double *dUL = base, *dUR = base + kNoAllocMax * 5, *dn = dUR + kNoAllocMax * 6, *dF = dn + kNoAllocMax * 3;
+  BES_HIP(hipMemcpy(dUL, UL, nn * nv * 8, hipMemcpyHostToDevice));
+  if (UR) {
+    BES_HIP(hipMemcpy(dUR, UR, nn * (nv + 1) * 8, hipMemcpyHostToDevice));
+  } else {   // slip walls read no right state; zero it so no stale values from an earlier call remain
+    const size_t cnt = nn * (nv + 1);
+    hipLaunchKernelGGL(k_zero_f64, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, 0, dUR, cnt);
+  }
+  BES_HIP(hipMemcpy(dn, normals, nn * ndim * 8, hipMemcpyHostToDevice));
+  GasConst gc = make_gas(gas);
+  const int bs = 128, nb = grid_for((int)n, bs);
+  if (ndim == 2) hipLaunchKernelGGL(k_flux_batch<2>, dim3(nb), dim3(bs), 0, 0, kind, (int)n, dUL, dUR, dn, gc, dF);
+  else hipLaunchKernelGGL(k_flux_batch<3>, dim3(nb), dim3(bs), 0, 0, kind, (int)n, dUL, dUR, dn, gc, dF);
+  BES_HIP(hipGetLastError());
+  BES_HIP(hipMemcpy(F, dF, nn * nv * 8, hipMemcpyDeviceToHost));
+  return BES_OK;
 }
 
 int bes_jacobian_batch(int ndim, int kind, long n, const double *UL, const double *UR,

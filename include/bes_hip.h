@@ -140,6 +140,16 @@ int bes_flux_batch(int ndim, int kind, long n, const double *UL, const double *U
                    const double *normals, const bes_gas *gas, double *F);
 int bes_jacobian_batch(int ndim, int kind, long n, const double *UL, const double *UR,
                        const double *normals, const bes_gas *gas, double *JL, double *JR);
+/* bes_flux_batch without any allocation: no counterpart in the reference; for
+ * callers that cannot allocate (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments and results as
+ * bes_flux_batch; inputs and outputs are staged through one fixed device array
+ * per GPU (held until the process ends), so n is at most BES_NOALLOC_MAX_FACES
+ * (BES_ERR_ARG above it).  Calls on one GPU share that array: calls that
+ * overlap in time on one GPU (several host threads) race. */
+enum { BES_NOALLOC_MAX_FACES = 16384 };
+int bes_flux_batch_noalloc(int ndim, int kind, long n, const double *UL, const double *UR,
+                           const double *normals, const bes_gas *gas, double *F);
 
 /* ---- implicit path: SolverImplicit + CoefficientMatrix + the AMGCL solve ----
  * bes_implicit_setup: allocate the matrix and solver workspace.
