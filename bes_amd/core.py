@@ -116,6 +116,10 @@ OPTIONAL_SIGS = {
     "bes_implicit_set_reorth": [C.c_void_p, C.c_double],
     "bes_implicit_stats": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],
     "bes_implicit_set_solver": [C.c_void_p, C.c_int],
+    "bes_implicit_set_coloring": [C.c_void_p, C.c_int],
+    "bes_autotune": [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p],
+    "bes_implicit_batch_state": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],
+    "bes_implicit_ncolors": [C.c_void_p],
     "bes_implicit_continue": [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p],
     "bes_set_batch": [C.c_void_p, C.c_int],
     "bes_get_batch": [C.c_void_p],
@@ -129,6 +133,7 @@ OPTIONAL_SIGS = {
     "bes_lp_loss_grad": [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p],
     "bes_lp_train": [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_int, C.c_ulonglong, C.c_void_p],
     "bes_lp_fallbacks": [C.c_void_p],
+    "bes_lp_set_online": [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_int],
     "bes_lp_set_rebuild_interval": [C.c_void_p, C.c_int],
     "bes_amg_set_params": [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int],
     "bes_amg_info": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int],
@@ -152,6 +157,8 @@ def lib():
             fn.argtypes = args
             fn.restype = C.c_int
         L.bes_implicit_nnz.argtypes = [C.c_void_p]
+        L.bes_lp_online_loss.argtypes = [C.c_void_p]
+        L.bes_lp_online_loss.restype = C.c_double
         L.bes_implicit_nnz.restype = C.c_long
         L.bes_last_error_face.argtypes = [C.c_void_p]
         L.bes_last_error_face.restype = C.c_long
@@ -306,6 +313,18 @@ class Context:
     def set_profiling(self, on=True):
         _check(lib().bes_set_profiling(self.h, int(on)))
 
+    def autotune(self, explicit=False, implicit=False, coloring=False):
+        """Block size per kernel kind (and the ILU(0) colouring) by timing on the
+        current data; returns {kind: (block size, microseconds)} and the colouring."""
+        flags = (1 if explicit else 0) | (2 if implicit else 0) | (4 if coloring else 0)
+        ch = np.zeros(16, np.int32)
+        us = np.zeros(16)
+        col = C.c_int(-1)
+        _check(lib().bes_autotune(self.h, flags, _p(ch), _p(us), C.byref(col)), self.h)
+        out = {PROF_KINDS[k]: (int(ch[k]), float(us[k])) for k in range(len(PROF_KINDS)) if us[k] > 0}
+        names = {v: k for k, v in self.COLORING.items()}
+        return out, names.get(col.value)
+
     def profile(self):
         n = len(PROF_KINDS)
         ms = np.zeros(n)
@@ -337,6 +356,14 @@ class Context:
         return {"reorths": r.value, "lp_fallbacks": f.value, "extra_restarts": e.value}
 
     SOLVER = {"gmres": 0, "fgmres": 0, "bicgstab": 1}
+    COLORING = {"morton": 0, "mesh": 1, "flow": 2}
+
+    def implicit_set_coloring(self, mode):
+        """Greedy visiting order of the ILU(0) colouring: morton, mesh or flow."""
+        _check(lib().bes_implicit_set_coloring(self.h, self.COLORING[mode]))
+
+    def implicit_ncolors(self):
+        return lib().bes_implicit_ncolors(self.h)
 
     def implicit_set_solver(self, solver):
         """Krylov method by the reference's LINEAR_SOLVER names (case-insensitive);
@@ -349,8 +376,9 @@ class Context:
         _check(lib().bes_implicit_set_precond(self.h, self.PRECOND[precond]))
 
     def implicit_assemble(self, cfl, local=True, repeat_nv=True):
-        lb = np.zeros(self.nv)
-        lr = np.zeros(self.nv)
+        """Linf of b and of sum F*A per variable ([batch][nv] for a batch)."""
+        lb = np.zeros(self._shape(self.nv))
+        lr = np.zeros(self._shape(self.nv))
         _check(lib().bes_implicit_assemble(self.h, cfl, int(local), int(repeat_nv), _p(lb), _p(lr)), self.h)
         return lb, lr
 
@@ -373,7 +401,7 @@ class Context:
         return it.value, rr.value
 
     def linear_solution(self):
-        x = np.zeros((self.ncell, self.nv))
+        x = np.zeros(self._shape(self.ncell, self.nv))
         _check(lib().bes_get_linear_solution(self.h, _p(x)))
         return x
 
@@ -384,11 +412,28 @@ class Context:
                      repeat_nv=True):
         """Backward-Euler steps; repeat_nv=True is the reference's nV-fold block repeat."""
         ex = np.ascontiguousarray(exponents, np.float64)
-        hist = np.zeros((nsteps, 2 * self.nv + 3))
+        hist = np.zeros(self._hist_shape(nsteps))
         done = C.c_int()
         _check(lib().bes_implicit_run(self.h, nsteps, cfl_min, cfl_max, _p(ex), int(local), int(repeat_nv),
                                       stop_drop, _p(hist), C.byref(done)), self.h)
         return hist[:done.value]
+
+    def _hist_shape(self, nsteps):
+        """Rows of the implicit loop's history: [nsteps][2nv+3], or [nsteps][batch][2nv+3]."""
+        w = 2 * self.nv + 3
+        return (nsteps, w) if self.batch == 1 else (nsteps, self.batch, w)
+
+    def implicit_batch_state(self):
+        """Per condition: steps taken, drop reached, next Courant number, and the
+        Krylov iterations and true relative residual of the last solve."""
+        B = self.batch
+        steps = np.zeros(B, np.int32)
+        reached = np.zeros(B, np.int32)
+        cfl = np.zeros(B)
+        iters = np.zeros(B, np.int32)
+        rr = np.zeros(B)
+        _check(lib().bes_implicit_batch_state(self.h, _p(steps), _p(reached), _p(cfl), _p(iters), _p(rr)))
+        return {"steps": steps, "reached": reached.astype(bool), "cfl": cfl, "iters": iters, "relres": rr}
 
     # ---- learned preconditioner ----
     def lp_init(self, seed=20261202):
@@ -424,6 +469,19 @@ class Context:
         hist = np.zeros(steps)
         _check(lib().bes_lp_train(self.h, steps, lr, nprobe, nrich, seed, _p(hist)), self.h)
         return hist
+
+    def lp_set_online(self, steps, lr=3e-3, nprobe=4, nrich=3):
+        """Adam steps on each assembled system before its solve (0: off)."""
+        _check(lib().bes_lp_set_online(self.h, int(steps), float(lr), int(nprobe), int(nrich)))
+
+    def lp_online_loss(self):
+        return float(lib().bes_lp_online_loss(self.h))
+
+    def lp_neutral(self):
+        """Decoder bias 0: every blend starts at 1/2, no hand-set prior."""
+        p = self.lp_params()
+        p[-32:] = 0.0
+        self.lp_set_params(p)
 
     def lp_fallbacks(self):
         return lib().bes_lp_fallbacks(self.h)
@@ -482,7 +540,7 @@ class Context:
 
     def implicit_continue(self, nsteps):
         """nsteps more steps of the last implicit_run; returns (history, reached)."""
-        hist = np.zeros((nsteps, 2 * self.nv + 3))
+        hist = np.zeros(self._hist_shape(nsteps))
         done = C.c_int()
         reached = C.c_int()
         _check(lib().bes_implicit_continue(self.h, nsteps, _p(hist), C.byref(done), C.byref(reached)), self.h)

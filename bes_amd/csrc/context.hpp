@@ -27,6 +27,8 @@ struct DevMesh {
   double *af = nullptr;     // [k][N] face area
   int *cfd = nullptr;       // [k][N] device face index of (cell, local face)
   uint32_t *info = nullptr; // [N] bits 0-7: face order (2 bits each); 8-11: owner flags
+  int *cfq = nullptr;       // [k][N] device face id of the cell's q-th face in reference face order
+  double *saq = nullptr;    // [k][N] that face's area, negated where the cell is not the owner
   // faces in device order: sorted by the owner's (Morton) index, then local face
   int F = 0;
   int *fo = nullptr;        // [F] owner cell
@@ -65,8 +67,9 @@ struct bes_ctx {
   bes::DevMesh dm;
   hipStream_t stream = nullptr;
   double *U0 = nullptr, *Ua = nullptr, *Ub = nullptr;  // [nv][N][B]
-  double *LG = nullptr;        // [N][nv*(1+nd)][B] per-cell pack: U, then phi*grad [v][d]
-  double *FF = nullptr;        // [F][nv][B] face fluxes per unit area (device face order)
+  double *LG = nullptr;        // [N][nv][1+nd][B] per-cell pack: per variable U_v, then phi*grad U_v [d]
+  double *FF = nullptr;        // [F][fs][B] face fluxes per unit area (device face order)
+  int fs = 0;                  // face-flux record length: nv, rounded up to even for B = 1 (16-byte access)
   std::vector<int64_t> fref;   // reference face id of each device face
   double *G = nullptr;         // [nv][nd][N][B] gradient (optional output)
   double *phi = nullptr;       // [nv][N][B] limiter (optional output)
@@ -99,6 +102,8 @@ struct bes_ctx {
   struct PendingEvent { int kind; double bytes; hipEvent_t a, b; };
   std::vector<PendingEvent> ev_pending;
   double prof_ms[kProfKinds] = {0};
+  // block size of each kernel kind (0: 256), set by bes_autotune
+  int tune_bs[kProfKinds] = {0};
   double prof_bytes[kProfKinds] = {0};
   long prof_n[kProfKinds] = {0};
 };
@@ -112,6 +117,7 @@ int hip_fail(hipError_t e, const char *what);
     if (e_ != hipSuccess) return bes::hip_fail(e_, #call);      \
   } while (0)
 inline int grid_for(int n, int bs) { return (n + bs - 1) / bs; }
+inline int tuned_bs(const bes_ctx *c, int kind) { return c->tune_bs[kind] > 0 ? c->tune_bs[kind] : 256; }
 // kernel kinds for per-kernel timing
 enum ProfKind { kProfTimestep = 0, kProfGradlim = 1, kProfResidual = 2, kProfJacobian = 3,
                 kProfAssemble = 4, kProfSpmv = 5, kProfPrecond = 6, kProfKrylov = 7, kProfFactor = 8,

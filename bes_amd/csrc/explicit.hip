@@ -86,9 +86,10 @@ int prof_collect(bes_ctx *c) {
 // geometry loads are wave-uniform and state loads are coalesced 512-B rows;
 // for B = 1 the layout is the plain SoA U[v][N].
 
+// cfl_b: per-condition Courant numbers when per_cond is set, else cfl for all
 template <int ND>
 __global__ __launch_bounds__(256) void k_timestep(const double *__restrict__ U, DevMesh dm, GasConst gc,
-                                                  double cfl, double *__restrict__ dt,
+                                                  double cfl, CoefB cfl_b, int per_cond, double *__restrict__ dt,
                                                   unsigned long long *__restrict__ gmin, int lgB) {
   constexpr int NV = ND + 2;
   const int N = dm.N, B = 1 << lgB;
@@ -104,7 +105,7 @@ __global__ __launch_bounds__(256) void k_timestep(const double *__restrict__ U, 
     double sr = 0.0;
 #pragma unroll
     for (int d = 0; d < ND; d++) sr += (c + ::fabs(g.velocity(u, d))) * dm.sproj[(size_t)d * N + i];
-    tv = cfl * dm.vol[i] / sr;
+    tv = (per_cond ? cfl_b.c[b] : cfl) * dm.vol[i] / sr;
     dt[(size_t)i * B + b] = tv;
   }
   if (gmin) {
@@ -130,104 +131,90 @@ __device__ __forceinline__ double face_limiter(double mn, double mx, double d2, 
 }
 
 // Green-Gauss gradient (GradientGreenGauss.cpp:21-135) fused with the
-// Venkatakrishnan limiter (LimiterVenkatakrishnan.cpp:21-99); writes the cell
-// pack [U | phi*grad] at ((i*PW + q)*B + b), PW = nv*(1+nd)
+// Venkatakrishnan limiter (LimiterVenkatakrishnan.cpp:21-99).  One wave per
+// conserved variable: a block of NV waves covers 64 (cell, condition) pairs
+// and wave v computes variable v for all of them, so a thread holds one
+// variable's gradient and bounds (few registers, many waves in flight); the
+// waves of a block read the same geometry records, from the CU's cache after
+// the first.  Every variable's arithmetic is the reference's, in its order.
+// Writes the cell pack group of variable v, [U_v | phi_v * grad U_v], at
+// ((i*NV + v)*GW + q)*B + b, GW = 1 + nd.
 template <int ND>
-__global__ __launch_bounds__(256) void k_gradlim(const double *__restrict__ U, DevMesh dm,
-                                                 double *__restrict__ PK, double *__restrict__ Gout,
-                                                 double *__restrict__ Pout, int lgB) {
-  constexpr int NV = ND + 2, K = ND + 1, PW = NV * (1 + ND);
+__global__ __launch_bounds__(64 * (ND + 2)) void k_gradlim(const double *__restrict__ U, DevMesh dm,
+                                                          double *__restrict__ PK, double *__restrict__ Gout,
+                                                          double *__restrict__ Pout, int lgB) {
+  constexpr int NV = ND + 2, K = ND + 1, GW = ND + 1;
   const int N = dm.N, B = 1 << lgB;
-  const int t = cell_index();
+  const int v = threadIdx.x >> 6;
+  const int t = xcd_block() * 64 + (threadIdx.x & 63);
   const int i = t >> lgB, b = t & (B - 1);
   if (i >= N) return;
-  double u[NV], mn[NV], mx[NV], g[NV][ND];
+  const double u = U[((size_t)v * N + i) * B + b];
+  double mn = u, mx = u, g[ND];
 #pragma unroll
-  for (int v = 0; v < NV; v++) {
-    u[v] = U[((size_t)v * N + i) * B + b];
-    mn[v] = u[v];
-    mx[v] = u[v];
-#pragma unroll
-    for (int d = 0; d < ND; d++) g[v][d] = 0.0;
-  }
+  for (int d = 0; d < ND; d++) g[d] = 0.0;
+  int nb[K];
 #pragma unroll
   for (int m = 0; m < K; m++) {
     const int j = dm.nbr[(size_t)m * N + i];
+    nb[m] = j;
     double S[ND];
 #pragma unroll
     for (int d = 0; d < ND; d++) S[d] = dm.Sg[((size_t)m * ND + d) * N + i];
     if (j >= 0) {
       const double w = dm.w[(size_t)m * N + i];
+      const double uj = U[((size_t)v * N + j) * B + b];
+      const double fv = w * u + (1 - w) * uj;
 #pragma unroll
-      for (int v = 0; v < NV; v++) {
-        const double uj = U[((size_t)v * N + j) * B + b];
-        const double fv = w * u[v] + (1 - w) * uj;
-#pragma unroll
-        for (int d = 0; d < ND; d++) g[v][d] += S[d] * fv;
-        mn[v] = (uj < mn[v]) ? uj : mn[v];
-        mx[v] = (uj > mx[v]) ? uj : mx[v];
-      }
+      for (int d = 0; d < ND; d++) g[d] += S[d] * fv;
+      mn = (uj < mn) ? uj : mn;
+      mx = (uj > mx) ? uj : mx;
     } else {
 #pragma unroll
-      for (int v = 0; v < NV; v++)
-#pragma unroll
-        for (int d = 0; d < ND; d++) g[v][d] += S[d] * (u[v]);
-      if (j == -1) {  // slip wall: tangential momentum on the face
-        double nn[ND], tg[ND], vn = 0;
+      for (int d = 0; d < ND; d++) g[d] += S[d] * (u);
+      if (j == -1 && v >= 1 && v <= ND) {   // slip wall: tangential momentum on the face
+        double nn[ND], vn = 0;
 #pragma unroll
         for (int d = 0; d < ND; d++) nn[d] = dm.nf[((size_t)m * ND + d) * N + i];
 #pragma unroll
-        for (int d = 0; d < ND; d++) vn += u[1 + d] * nn[d];
+        for (int d = 0; d < ND; d++) vn += U[((size_t)(1 + d) * N + i) * B + b] * nn[d];
+        const double tg = u - vn * nn[v - 1];
 #pragma unroll
-        for (int d = 0; d < ND; d++) tg[d] = u[1 + d] - vn * nn[d];
-#pragma unroll
-        for (int a = 0; a < ND; a++)
-#pragma unroll
-          for (int d = 0; d < ND; d++) g[1 + a][d] += (tg[a] * S[d] - u[1 + a] * S[d]);
+        for (int d = 0; d < ND; d++) g[d] += (tg * S[d] - u * S[d]);
       }
     }
   }
   const double V = dm.vol[i];
 #pragma unroll
-  for (int v = 0; v < NV; v++)
-#pragma unroll
-    for (int d = 0; d < ND; d++) g[v][d] /= V;
+  for (int d = 0; d < ND; d++) g[d] /= V;
   const double eps2 = dm.eps2[i];
+  const double dmin = mn - u, dmax = mx - u;
+  double best = 1.0;
 #pragma unroll
-  for (int v = 0; v < NV; v++) {
-    const double dmin = mn[v] - u[v], dmax = mx[v] - u[v];
-    double best = 1.0;
+  for (int m = 0; m < K; m++) {
+    if (nb[m] < 0) continue;
+    double d2 = 0;
 #pragma unroll
-    for (int m = 0; m < K; m++) {
-      if (dm.nbr[(size_t)m * N + i] < 0) continue;
-      double d2 = 0;
-#pragma unroll
-      for (int d = 0; d < ND; d++) d2 += g[v][d] * dm.rl[((size_t)m * ND + d) * N + i];
-      d2 /= 2.0;
-      const double f = face_limiter(dmin, dmax, d2, eps2);
-      best = (f < best) ? f : best;
-    }
-#pragma unroll
-    for (int d = 0; d < ND; d++) {
-      if (Gout) Gout[(((size_t)v * ND + d) * N + i) * B + b] = g[v][d];
-      g[v][d] = best * g[v][d];   // phi*grad: the product the reconstruction multiplies
-    }
-    if (Pout) Pout[((size_t)v * N + i) * B + b] = best;
+    for (int d = 0; d < ND; d++) d2 += g[d] * dm.rl[((size_t)m * ND + d) * N + i];
+    d2 /= 2.0;
+    const double f = face_limiter(dmin, dmax, d2, eps2);
+    best = (f < best) ? f : best;
   }
-  double pk[PW];
+  double pk[GW];
+  pk[0] = u;
 #pragma unroll
-  for (int v = 0; v < NV; v++) {
-    pk[v] = u[v];
-#pragma unroll
-    for (int d = 0; d < ND; d++) pk[NV + v * ND + d] = g[v][d];
+  for (int d = 0; d < ND; d++) {
+    if (Gout) Gout[(((size_t)v * ND + d) * N + i) * B + b] = g[d];
+    pk[1 + d] = best * g[d];   // phi*grad: the product the reconstruction multiplies
   }
-  if (B == 1) {   // one contiguous run per cell: 16-byte stores
-    double2 *dst = reinterpret_cast<double2 *>(PK + (size_t)i * PW);
+  if (Pout) Pout[((size_t)v * N + i) * B + b] = best;
+  if (B == 1 && GW % 2 == 0) {   // the group is one aligned 32-byte run: 16-byte stores
+    double2 *dst = reinterpret_cast<double2 *>(PK + ((size_t)i * NV + v) * GW);
 #pragma unroll
-    for (int q = 0; q < PW / 2; q++) dst[q] = make_double2(pk[2 * q], pk[2 * q + 1]);
-  } else {        // condition-fastest: each store a coalesced row across the wave
+    for (int q = 0; q < GW / 2; q++) dst[q] = make_double2(pk[2 * q], pk[2 * q + 1]);
+  } else {
 #pragma unroll
-    for (int q = 0; q < PW; q++) PK[((size_t)i * PW + q) * B + b] = pk[q];
+    for (int q = 0; q < GW; q++) PK[(((size_t)i * NV + v) * GW + q) * B + b] = pk[q];
   }
 }
 
@@ -235,7 +222,7 @@ __global__ __launch_bounds__(256) void k_gradlim(const double *__restrict__ U, D
 template <int ND>
 __device__ __forceinline__ void face_state(const double *__restrict__ PK, int cell, int b, int B,
                                            const double *r, double *uf) {
-  constexpr int NV = ND + 2, PW = NV * (1 + ND);
+  constexpr int NV = ND + 2, GW = ND + 1, PW = NV * GW;
   double pk[PW];
   if (B == 1) {
     const double2 *src = reinterpret_cast<const double2 *>(PK + (size_t)cell * PW);
@@ -251,27 +238,29 @@ __device__ __forceinline__ void face_state(const double *__restrict__ PK, int ce
   }
 #pragma unroll
   for (int v = 0; v < NV; v++) {
-    uf[v] = pk[v];
+    uf[v] = pk[v * GW];
 #pragma unroll
-    for (int d = 0; d < ND; d++) uf[v] += pk[NV + v * ND + d] * r[d];
+    for (int d = 0; d < ND; d++) uf[v] += pk[v * GW + 1 + d] * r[d];
   }
 }
 
 // One thread per (face, condition), faces in device order: second-order states
 // of both sides from the cell packs, then the face flux per unit area
 // (FluxRoe.tpp:306-333, FluxFreestream.tpp:192-226, FluxSlipWall.tpp:32-53),
-// computed once per face.  finf: freestream primitive state per condition.
+// computed once per face and stored as a record of fs values (nv, padded to an
+// even count when B = 1 so the record moves in 16-byte pieces).  finf:
+// freestream primitive state per condition.
 template <int ND>
 __global__ __launch_bounds__(256) void k_face_flux(const double *__restrict__ PK, DevMesh dm, GasConst gc,
                                                    const double *__restrict__ finf, double *__restrict__ FF,
-                                                   unsigned long long *__restrict__ err, int lgB) {
+                                                   int fs, unsigned long long *__restrict__ err, int lgB) {
   constexpr int NV = ND + 2;
   const int F = dm.F, B = 1 << lgB;
   const int t = xcd_block() * blockDim.x + threadIdx.x;
   const int f = t >> lgB, b = t & (B - 1);
   if (f >= F) return;
   const int o = dm.fo[f], j = dm.fnb[f];
-  double nn[ND], rs[ND], ul[NV], out[NV];
+  double nn[ND], rs[ND], ul[NV], out[NV + 1];
 #pragma unroll
   for (int d = 0; d < ND; d++) {
     nn[d] = dm.fnr[(size_t)d * F + f];
@@ -296,18 +285,27 @@ __global__ __launch_bounds__(256) void k_face_flux(const double *__restrict__ PK
 #pragma unroll
   for (int v = 0; v < NV; v++) bad |= (out[v] != out[v]);
   if (bad) atomicMin(err, (unsigned long long)f);
+  out[NV] = 0.0;
+  if (B == 1 && fs % 2 == 0) {
+    double2 *dst = reinterpret_cast<double2 *>(FF + (size_t)f * fs);
 #pragma unroll
-  for (int v = 0; v < NV; v++) FF[((size_t)f * NV + v) * B + b] = out[v];
+    for (int q = 0; q < (NV + 1) / 2; q++) dst[q] = make_double2(out[2 * q], out[2 * q + 1]);
+  } else {
+#pragma unroll
+    for (int v = 0; v < NV; v++) FF[((size_t)f * fs + v) * B + b] = out[v];
+  }
 }
 
 // One thread per (cell, condition): residual sum_f +-F*A over the cell's faces
 // in the reference face order (SolverExplicit.cpp:325-341), fused with the
 // stage update U = U0 - alpha*dt/V*R (:364-375) and the Linf monitor (:356-362).
+// The faces come pre-sorted in that order with the area negated where the cell
+// is the neighbour (R + x*(-A) is R - x*A exactly).
 template <int ND>
 __global__ __launch_bounds__(256) void k_cell_residual(
     // U0 and Ud may be the same array (the last stage updates in place: each
     // thread reads only its own U0 entry before writing it), so no restrict
-    const double *__restrict__ FF, const double *U0, double *Ud, const double *__restrict__ dt,
+    const double *__restrict__ FF, int fs, const double *U0, double *Ud, const double *__restrict__ dt,
     const double *__restrict__ dtg, double alpha, DevMesh dm, double *__restrict__ Rout,
     unsigned long long *__restrict__ linf_bits, int lgB) {
   constexpr int NV = ND + 2, K = ND + 1;
@@ -318,19 +316,30 @@ __global__ __launch_bounds__(256) void k_cell_residual(
 #pragma unroll
   for (int v = 0; v < NV; v++) R[v] = 0.0;
   if (i < N) {
-    const uint32_t info = dm.info[i];
+    int fq[K];
+    double aq[K];
 #pragma unroll
     for (int q = 0; q < K; q++) {
-      const int m = (info >> (2 * q)) & 3;
-      const bool own = (info >> (8 + m)) & 1;
-      const int f = dm.cfd[(size_t)m * N + i];
-      const double A = dm.af[(size_t)m * N + i];
+      fq[q] = dm.cfq[(size_t)q * N + i];
+      aq[q] = dm.saq[(size_t)q * N + i];
+    }
 #pragma unroll
-      for (int v = 0; v < NV; v++) {
-        const double x = FF[((size_t)f * NV + v) * B + b];
-        if (own) R[v] += x * A;
-        else R[v] -= x * A;
+    for (int q = 0; q < K; q++) {
+      double x[NV + 1];
+      if (B == 1 && fs % 2 == 0) {
+        const double2 *src = reinterpret_cast<const double2 *>(FF + (size_t)fq[q] * fs);
+#pragma unroll
+        for (int p = 0; p < (NV + 1) / 2; p++) {
+          const double2 tt = src[p];
+          x[2 * p] = tt.x;
+          x[2 * p + 1] = tt.y;
+        }
+      } else {
+#pragma unroll
+        for (int v = 0; v < NV; v++) x[v] = FF[((size_t)fq[q] * fs + v) * B + b];
       }
+#pragma unroll
+      for (int v = 0; v < NV; v++) R[v] += x[v] * aq[q];
     }
     if (Ud) {
       const double c = alpha * (dtg ? dtg[b] : dt[(size_t)i * B + b]) / dm.vol[i];
@@ -462,7 +471,8 @@ static int build_device_mesh(bes_ctx *c) {
   for (int i = 0; i < N; i++) c->iperm[c->perm[i]] = i;
   std::vector<double> vol(N), eps2(N), sproj((size_t)nd * N), Sg((size_t)k * nd * N), w((size_t)k * N),
       rl((size_t)k * nd * N), nf((size_t)k * nd * N), af((size_t)k * N);
-  std::vector<int> nbr((size_t)k * N), cfd((size_t)k * N);
+  std::vector<int> nbr((size_t)k * N), cfd((size_t)k * N), cfq((size_t)k * N);
+  std::vector<double> saq((size_t)k * N);
   std::vector<uint32_t> info(N);
   // faces in device order: by owner (Morton index), then the owner's local face
   const int F = (int)g.nface;
@@ -512,6 +522,11 @@ static int build_device_mesh(bes_ctx *c) {
     std::sort(order, order + k, [&](int a, int b) { return g.cface[e * k + a] < g.cface[e * k + b]; });
     uint32_t bits = 0;
     for (int q = 0; q < k; q++) bits |= (uint32_t)order[q] << (2 * q);
+    for (int q = 0; q < k; q++) {   // faces in reference order, signed area
+      const int64_t f = g.cface[e * k + order[q]];
+      cfq[(size_t)q * N + i] = ref2dev[f];
+      saq[(size_t)q * N + i] = g.owner[f] == e ? g.area[f] : -g.area[f];
+    }
     for (int m = 0; m < k; m++) {
       const int64_t s = e * k + m, f = g.cface[s];
       const bool own = g.owner[f] == e;
@@ -539,7 +554,8 @@ static int build_device_mesh(bes_ctx *c) {
   if ((s = upload(&dm.vol, vol)) || (s = upload(&dm.eps2, eps2)) || (s = upload(&dm.sproj, sproj)) ||
       (s = upload(&dm.nbr, nbr)) || (s = upload(&dm.Sg, Sg)) || (s = upload(&dm.w, w)) ||
       (s = upload(&dm.rl, rl)) || (s = upload(&dm.nf, nf)) || (s = upload(&dm.af, af)) ||
-      (s = upload(&dm.cfd, cfd)) || (s = upload(&dm.info, info)) || (s = upload(&dm.fo, fo)) ||
+      (s = upload(&dm.cfd, cfd)) || (s = upload(&dm.info, info)) || (s = upload(&dm.cfq, cfq)) ||
+      (s = upload(&dm.saq, saq)) || (s = upload(&dm.fo, fo)) ||
       (s = upload(&dm.fnb, fnb)) || (s = upload(&dm.fnr, fnr)) || (s = upload(&dm.frs, frs)) ||
       (s = upload(&dm.frn, frn)))
     return s;
@@ -575,9 +591,10 @@ static int create_from_input(const MeshInput &in, int geom_mode, const bes_gas *
   hipError_t he = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (he != hipSuccess) { s = hip_fail(he, "hipStreamCreate"); delete c; return s; }
   const size_t N = c->N, nv = c->nv, nd = c->nd;
+  c->fs = (int)((nv + 1) / 2 * 2);
   if ((s = build_device_mesh(c)) || (s = dalloc(&c->U0, nv * N)) || (s = dalloc(&c->Ua, nv * N)) ||
       (s = dalloc(&c->Ub, nv * N)) || (s = dalloc(&c->LG, nv * (1 + nd) * N)) ||
-      (s = dalloc(&c->FF, nv * (size_t)c->dm.F)) || (s = dalloc(&c->dt, N)) ||
+      (s = dalloc(&c->FF, (size_t)c->fs * c->dm.F)) || (s = dalloc(&c->dt, N)) ||
       (s = dalloc(&c->dtg, bes_ctx::kMaxBatch)) || (s = dalloc(&c->red, bes_ctx::kRedWords)) ||
       (s = dalloc(&c->err, 1)) || (s = dalloc(&c->dfinf, bes_ctx::kMaxBatch * 6)) ||
       (s = dalloc(&c->dit, 1))) {
@@ -614,8 +631,11 @@ static int check_err(bes_ctx *c) {
 }
 
 // launch helpers shared with the implicit path
-int launch_timestep(bes_ctx *c, const double *U, double cfl, int local) {
-  const int bs = 256, B = c->B, nb = grid_for(c->N * B, bs);
+int launch_timestep(bes_ctx *c, const double *U, double cfl, int local, const double *cfl_b = nullptr) {
+  const int bs = tuned_bs(c, kProfTimestep), B = c->B, nb = grid_for(c->N * B, bs);
+  CoefB cb;
+  for (int b = 0; b < B; b++) cb.c[b] = cfl_b ? cfl_b[b] : cfl;
+  const int pc = cfl_b ? 1 : 0;
   unsigned long long *gm = nullptr;
   if (!local) {
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, c->stream, c->red, B, 0x7ff0000000000000ull);
@@ -624,9 +644,11 @@ int launch_timestep(bes_ctx *c, const double *U, double cfl, int local) {
   {
     ProfScope ps(c, kProfTimestep, (double)c->N * (8.0 * (c->nv * B + c->nd + 1) + 8.0 * B));
     if (c->nd == 2)
-      hipLaunchKernelGGL(k_timestep<2>, dim3(nb), dim3(bs), 0, c->stream, U, c->dm, c->gc, cfl, c->dt, gm, c->lgB);
+      hipLaunchKernelGGL(k_timestep<2>, dim3(nb), dim3(bs), 0, c->stream, U, c->dm, c->gc, cfl, cb, pc, c->dt, gm,
+                         c->lgB);
     else
-      hipLaunchKernelGGL(k_timestep<3>, dim3(nb), dim3(bs), 0, c->stream, U, c->dm, c->gc, cfl, c->dt, gm, c->lgB);
+      hipLaunchKernelGGL(k_timestep<3>, dim3(nb), dim3(bs), 0, c->stream, U, c->dm, c->gc, cfl, cb, pc, c->dt, gm,
+                         c->lgB);
   }
   if (!local) hipLaunchKernelGGL(k_decode_min, dim3(1), dim3(64), 0, c->stream, c->red, c->dtg, B);
   BES_HIP(hipGetLastError());
@@ -639,42 +661,45 @@ double gradlim_bytes(const bes_ctx *c) {
   return N * (8 * nv * B + k * (4 + 16 * nd + 8) + 16 + 8 * nv * (1 + nd) * B);
 }
 
+// one block per 64 (cell, condition) pairs, one wave per conserved variable
 int launch_gradlim(bes_ctx *c, const double *U, double *Gout, double *Pout) {
-  const int bs = 256, nb = grid_for(c->N * c->B, bs);
+  const int nb = grid_for(c->N * c->B, 64);
   ProfScope ps(c, kProfGradlim, gradlim_bytes(c));
   if (c->nd == 2)
-    hipLaunchKernelGGL(k_gradlim<2>, dim3(nb), dim3(bs), 0, c->stream, U, c->dm, c->LG, Gout, Pout, c->lgB);
+    hipLaunchKernelGGL(k_gradlim<2>, dim3(nb), dim3(64 * 4), 0, c->stream, U, c->dm, c->LG, Gout, Pout, c->lgB);
   else
-    hipLaunchKernelGGL(k_gradlim<3>, dim3(nb), dim3(bs), 0, c->stream, U, c->dm, c->LG, Gout, Pout, c->lgB);
+    hipLaunchKernelGGL(k_gradlim<3>, dim3(nb), dim3(64 * 5), 0, c->stream, U, c->dm, c->LG, Gout, Pout, c->lgB);
   BES_HIP(hipGetLastError());
   return BES_OK;
 }
 
 // face flux: face records, both cell packs (each pack read once overall), fluxes
 double faceflux_bytes(const bes_ctx *c) {
-  const double F = c->dm.F, N = c->N, B = c->B, nv = c->nv, nd = c->nd;
-  return F * (8 + 24 * nd + 8 * nv * B) + N * 8 * nv * (1 + nd) * B;
+  const double F = c->dm.F, N = c->N, B = c->B, nd = c->nd, nv = c->nv;
+  return F * (8 + 24 * nd + 8 * c->fs * B) + N * 8 * nv * (1 + nd) * B;
 }
 
 // face fluxes of the current packs (shared with the implicit assembly)
 int launch_face_flux(bes_ctx *c) {
-  const int bs = 256, nb = grid_for(c->dm.F * c->B, bs);
+  const int bs = tuned_bs(c, kProfFaceFlux), nb = grid_for(c->dm.F * c->B, bs);
   const int nd = c->nd;
   ProfScope ps(c, kProfFaceFlux, faceflux_bytes(c));
   if (nd == 2)
     hipLaunchKernelGGL(k_face_flux<2>, dim3(nb), dim3(bs), 0, c->stream, c->LG, c->dm, c->gc, c->dfinf, c->FF,
-                       c->err, c->lgB);
+                       c->fs, c->err, c->lgB);
   else
     hipLaunchKernelGGL(k_face_flux<3>, dim3(nb), dim3(bs), 0, c->stream, c->LG, c->dm, c->gc, c->dfinf, c->FF,
-                       c->err, c->lgB);
+                       c->fs, c->err, c->lgB);
   BES_HIP(hipGetLastError());
   return BES_OK;
 }
 
+// per cell: ordered face ids and signed areas, dt, volume, U0 and U; the face
+// flux records once
 double residual_bytes(const bes_ctx *c, bool update, bool rout) {
   const double N = c->N, B = c->B, nv = c->nv, k = c->k;
-  return N * (4 + k * 12 + (update ? 8 + 8 * B + 16 * nv * B : 0) + (rout ? 8 * nv * B : 0)) +
-         (double)c->dm.F * 8 * nv * B;
+  return N * (k * 12 + (update ? 8 + 8 * B + 16 * nv * B : 0) + (rout ? 8 * nv * B : 0)) +
+         (double)c->dm.F * 8 * c->fs * B;
 }
 
 // residual of the packs of the last gradient pass; with U0/Ud the stage update
@@ -682,15 +707,15 @@ int launch_residual(bes_ctx *c, const double *U0, double *Ud, double alpha, int 
                     unsigned long long *linf_bits) {
   int s = launch_face_flux(c);
   if (s) return s;
-  const int bs = 256, nb = grid_for(c->N * c->B, bs);
+  const int bs = tuned_bs(c, kProfResidual), nb = grid_for(c->N * c->B, bs);
   const double *dtg = local ? nullptr : c->dtg;
   ProfScope ps(c, kProfResidual, residual_bytes(c, U0 != nullptr, Rout != nullptr));
   if (c->nd == 2)
-    hipLaunchKernelGGL(k_cell_residual<2>, dim3(nb), dim3(bs), 0, c->stream, c->FF, U0, Ud, c->dt, dtg, alpha, c->dm,
-                       Rout, linf_bits, c->lgB);
+    hipLaunchKernelGGL(k_cell_residual<2>, dim3(nb), dim3(bs), 0, c->stream, c->FF, c->fs, U0, Ud, c->dt, dtg, alpha,
+                       c->dm, Rout, linf_bits, c->lgB);
   else
-    hipLaunchKernelGGL(k_cell_residual<3>, dim3(nb), dim3(bs), 0, c->stream, c->FF, U0, Ud, c->dt, dtg, alpha, c->dm,
-                       Rout, linf_bits, c->lgB);
+    hipLaunchKernelGGL(k_cell_residual<3>, dim3(nb), dim3(bs), 0, c->stream, c->FF, c->fs, U0, Ud, c->dt, dtg, alpha,
+                       c->dm, Rout, linf_bits, c->lgB);
   BES_HIP(hipGetLastError());
   return BES_OK;
 }
@@ -742,8 +767,9 @@ static int resize_batch(bes_ctx *c, int B) {
   while ((1 << c->lgB) < B) c->lgB++;
   const size_t N = c->N, nv = c->nv, nd = c->nd, Bs = B;
   int s;
+  c->fs = B == 1 ? (int)((nv + 1) / 2 * 2) : (int)nv;
   if ((s = dalloc(&c->U0, nv * N * Bs)) || (s = dalloc(&c->Ua, nv * N * Bs)) || (s = dalloc(&c->Ub, nv * N * Bs)) ||
-      (s = dalloc(&c->LG, nv * (1 + nd) * N * Bs)) || (s = dalloc(&c->FF, nv * (size_t)c->dm.F * Bs)) ||
+      (s = dalloc(&c->LG, nv * (1 + nd) * N * Bs)) || (s = dalloc(&c->FF, (size_t)c->fs * c->dm.F * Bs)) ||
       (s = dalloc(&c->dt, N * Bs)))
     return s;
   std::vector<double> fb((size_t)B * (nv + 1));
@@ -811,7 +837,8 @@ int bes_ctx_destroy(bes_ctx *c) {
   if (c->stream) hipStreamSynchronize(c->stream);
   bes_implicit_free(c);
   void *ptrs[] = {c->dm.vol, c->dm.eps2, c->dm.sproj, c->dm.nbr, c->dm.Sg, c->dm.w, c->dm.rl, c->dm.nf,
-                  c->dm.af, c->dm.cfd, c->dm.info, c->dm.fo, c->dm.fnb, c->dm.fnr, c->dm.frs, c->dm.frn,
+                  c->dm.af, c->dm.cfd, c->dm.info, c->dm.cfq, c->dm.saq, c->dm.fo, c->dm.fnb, c->dm.fnr,
+                  c->dm.frs, c->dm.frn,
                   c->FF, c->U0, c->Ua, c->Ub, c->LG, c->G, c->phi,
                   c->dt, c->dtg, c->R, c->red, c->err, c->linf, c->dfinf, c->dit};
   drop_graph(c);
@@ -870,11 +897,8 @@ int bes_set_batch(bes_ctx *c, int nbatch) {
     set_error("mesh too large for this batch size");
     return BES_ERR_ARG;
   }
-  if (nbatch > 1 && c->impl) {
-    set_error("the implicit path runs one condition; release it (bes_implicit_setup) first");
-    return BES_ERR_ARG;
-  }
   if (nbatch == c->B) return BES_OK;
+  bes_implicit_free(c);   // the implicit workspace is sized for one batch size
   return resize_batch(c, nbatch);
 }
 

@@ -37,7 +37,8 @@ struct Impl {
   double rtol = 1e-7;
   int repeat_nv = 1;
   FaceDev fd;
-  double *JF = nullptr;    // [2][nv*nv][F] face Jacobian pairs, entry-major (device face order)
+  void *JF = nullptr;      // [F][2][JP] face Jacobian pairs (TM), one padded row-major record per
+                           // block, device face order
   double *b = nullptr;     // [N][nv] FP64 right-hand side
   double *x = nullptr;     // [N][nv] FP64 solution
   void *A = nullptr;       // matrix values (TM)
@@ -46,6 +47,7 @@ struct Impl {
   double *xc = nullptr;    // solution in colour order
   void *LU = nullptr;      // factor / inverse diagonal blocks (TM), [(k+1)*nv*nv][N]
   void *Dr = nullptr;      // ILU(0): inverse diagonal blocks again, row-major [N][nv*nv] (TM)
+  void *Ur = nullptr;      // ILU(0): upper blocks of each row, row-major [N][k][nv*nv] by position (TM)
   void *LM = nullptr;      // learned operator blocks (TM), [(k+1)*nv*nv][N] (learned.hip)
   __half *LMh = nullptr;   // FP16 copy applied by the FP32 solves (the operator's precision
                            // sets the convergence rate only, never the solution)
@@ -55,6 +57,11 @@ struct Impl {
   int lp_since = 0;        // solves since its last build
   bool lp_current = false; // the stored operator belongs to the present parameters
   long lp_builds = 0;
+  // online training: Adam steps on each assembled system before its solve
+  int lp_online_steps = 0, lp_online_nprobe = 4, lp_online_nrich = 3;
+  double lp_online_lr = 3e-3;
+  unsigned long long lp_online_seed = 1;
+  double lp_online_loss = 0;   // loss of the last online step
   void *amg = nullptr;     // aggregation AMG hierarchy (amg.hip)
   void *V = nullptr;       // Krylov basis (TV) [(m+1)][vstride]
   void *Z = nullptr;       // work vectors (TV) [3][vstride]
@@ -62,12 +69,15 @@ struct Impl {
   double reorth_eta = 0.7; // DGKS re-orthogonalisation when ||w|| drops below eta of its norm
   long reorths = 0;        // re-orthogonalisations so far
   long extra_restarts = 0; // GMRES cycles whose Givens estimate met the target but the true residual did not
-  double *part = nullptr;  // reduction partials [32][kRedBlocks]
-  double *dots = nullptr;  // [64]
-  double *hdots = nullptr; // [64] pinned host copy of dots (Arnoldi step)
+  double *part = nullptr;  // reduction partials [32][kRedBlocks] (BiCGStab)
+  int lanes = 0;           // lanes L of the deterministic batched reductions
+  double *S = nullptr;     // their lane sums [9][B][L]
+  double *dots = nullptr;  // [64][B] reduced dot products, condition fastest
+  double *hdots = nullptr; // [64][B] pinned host copy of dots (Arnoldi step)
   hipEvent_t ev_dots = nullptr;  // recorded after the copy into hdots
-  unsigned long long *bits = nullptr;   // [16] Linf monitors
+  unsigned long long *bits = nullptr;   // [2][8][B] Linf monitors of b and sum F*A, per variable and condition
   int ncolor = 0;
+  int coloring = 0;               // greedy visiting order of the ILU(0) colouring (bes_implicit_set_coloring)
   std::vector<int> color_start;   // host: rows of colour c are crow[color_start[c] .. color_start[c+1])
   int *crow = nullptr;            // [N] rows grouped by colour: crow[p] = cell at position p
   int *color = nullptr;           // [N]
@@ -79,13 +89,19 @@ struct Impl {
   long ninternal = 0;              // internal faces (off-diagonal block pairs)
   int last_iters = 0;
   double last_relres = 0;
+  std::vector<int> iters_b;       // per condition: Krylov iterations of the last solve
+  std::vector<double> relres_b;   // and its true relative residual
+  std::vector<char> skip_b;       // conditions the next solve leaves alone (x = 0)
   bool assembled = false, factored = false;
   // pseudo-time loop state (bes_implicit_run / bes_implicit_continue)
   struct Run {
-    bool active = false, reached = false;
-    double cfl = 0, cfl_min = 0, cfl_max = 0, ex[2] = {1, 1}, stop_drop = 0, r0 = -1;
+    bool active = false, reached = false;   // reached: every condition reached its drop
+    double cfl_min = 0, cfl_max = 0, ex[2] = {1, 1}, stop_drop = 0;
     int local = 1, repeat_nv = 1;
-    std::vector<double> lb_old;
+    std::vector<double> cfl, r0;      // [B] Courant number of the next step, first density residual
+    std::vector<double> lb_old;       // [B][nv] Linf of b of the previous step
+    std::vector<char> done;           // [B] reached its drop (state frozen from then on)
+    std::vector<int> steps;           // [B] steps taken
   } run;
 };
 
@@ -147,6 +163,20 @@ __device__ __forceinline__ void store_block(TM *__restrict__ base, int slot, int
 #pragma unroll
   for (int e = 0; e < NB; e++) base[((size_t)slot * NB + e) * N + i] = (TM)B[e];
 }
+// the same for a batch of conditions (condition b fastest): entry e of slot s
+// of row i at ((s*NB + e)*N + i)*nb + b; nb = 1 gives the layouts above
+template <typename TM, int NB>
+__device__ __forceinline__ void load_block_b(const TM *__restrict__ base, int slot, int N, int i, int nb, int b,
+                                             double *B) {
+#pragma unroll
+  for (int e = 0; e < NB; e++) B[e] = (double)base[(((size_t)slot * NB + e) * N + i) * nb + b];
+}
+template <typename TM, int NB>
+__device__ __forceinline__ void store_block_b(TM *__restrict__ base, int slot, int N, int i, int nb, int b,
+                                              const double *B) {
+#pragma unroll
+  for (int e = 0; e < NB; e++) base[(((size_t)slot * NB + e) * N + i) * nb + b] = (TM)B[e];
+}
 
 __device__ __forceinline__ double to_d(double x) { return x; }
 __device__ __forceinline__ double to_d(float x) { return (double)x; }
@@ -155,22 +185,25 @@ __device__ __forceinline__ double to_d(__half x) { return (double)__half2float(x
 // ---------------------------------------------------------------------------
 // y = A x for an ELL-BSR matrix (vectors [N][nv], type TV)
 // ---------------------------------------------------------------------------
+// y = A x for B = 1 << lgB conditions at once: matrix entries at
+// ((slot*NB + e)*N + i)*B + b, vectors at (i*nv + v)*B + b (condition fastest)
 template <int ND, typename TM, typename TV>
 __global__ __launch_bounds__(256) void k_spmv(const TM *__restrict__ A, DevMesh dm, const TV *__restrict__ x,
-                                              TV *__restrict__ y) {
+                                              TV *__restrict__ y, int lgB) {
   constexpr int NV = ND + 2, K = ND + 1, NB = NV * NV;
-  const int N = dm.N;
-  const int i = cell_index();
+  const int N = dm.N, B = 1 << lgB;
+  const int t = cell_index();
+  const int i = t >> lgB, b = t & (B - 1);
   if (i >= N) return;
   double acc[NV];
   double xi[NV];
 #pragma unroll
-  for (int c = 0; c < NV; c++) xi[c] = (double)x[(size_t)i * NV + c];
+  for (int c = 0; c < NV; c++) xi[c] = (double)x[((size_t)i * NV + c) * B + b];
 #pragma unroll
   for (int r = 0; r < NV; r++) {
     double s = 0;
 #pragma unroll
-    for (int c = 0; c < NV; c++) s += to_d(A[((size_t)(r * NV + c)) * N + i]) * xi[c];
+    for (int c = 0; c < NV; c++) s += to_d(A[((size_t)(r * NV + c) * N + i) * B + b]) * xi[c];
     acc[r] = s;
   }
 #pragma unroll
@@ -179,18 +212,18 @@ __global__ __launch_bounds__(256) void k_spmv(const TM *__restrict__ A, DevMesh 
     if (j < 0) continue;
     double xj[NV];
 #pragma unroll
-    for (int c = 0; c < NV; c++) xj[c] = (double)x[(size_t)j * NV + c];
-    const TM *Ao = A + (size_t)(1 + m) * NB * N;
+    for (int c = 0; c < NV; c++) xj[c] = (double)x[((size_t)j * NV + c) * B + b];
+    const TM *Ao = A + (size_t)(1 + m) * NB * N * B;
 #pragma unroll
     for (int r = 0; r < NV; r++) {
       double s = 0;
 #pragma unroll
-      for (int c = 0; c < NV; c++) s += to_d(Ao[((size_t)(r * NV + c)) * N + i]) * xj[c];
+      for (int c = 0; c < NV; c++) s += to_d(Ao[((size_t)(r * NV + c) * N + i) * B + b]) * xj[c];
       acc[r] += s;
     }
   }
 #pragma unroll
-  for (int r = 0; r < NV; r++) y[(size_t)i * NV + r] = (TV)acc[r];
+  for (int r = 0; r < NV; r++) y[((size_t)i * NV + r) * B + b] = (TV)acc[r];
 }
 
 }  // namespace bes

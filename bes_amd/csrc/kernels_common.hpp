@@ -18,6 +18,11 @@ __device__ __forceinline__ int xcd_block() {
 
 __device__ __forceinline__ int cell_index() { return xcd_block() * blockDim.x + threadIdx.x; }
 
+// one value per condition of a batch (at most 64), passed to a kernel by value
+struct CoefB {
+  double c[64];
+};
+
 __device__ __forceinline__ double wave_max(double v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));

@@ -1006,8 +1006,8 @@ static double read_sumsq(bes_ctx *c, Lp *L, const float *x, size_t n, int *st) {
 template <typename TM>
 static void spmv_f(bes_ctx *c, const TM *A, const float *x, float *y) {
   const int bs = 256;
-  if (c->nd == 2) hipLaunchKernelGGL((k_spmv<2, TM, float>), dim3(grid_for(c->N, bs)), dim3(bs), 0, c->stream, A, c->dm, x, y);
-  else hipLaunchKernelGGL((k_spmv<3, TM, float>), dim3(grid_for(c->N, bs)), dim3(bs), 0, c->stream, A, c->dm, x, y);
+  if (c->nd == 2) hipLaunchKernelGGL((k_spmv<2, TM, float>), dim3(grid_for(c->N, bs)), dim3(bs), 0, c->stream, A, c->dm, x, y, 0);
+  else hipLaunchKernelGGL((k_spmv<3, TM, float>), dim3(grid_for(c->N, bs)), dim3(bs), 0, c->stream, A, c->dm, x, y, 0);
 }
 template <typename TM>
 static void spmv_t_f(bes_ctx *c, Impl *im, const TM *A, const float *x, float *y) {
@@ -1243,6 +1243,24 @@ int bes_lp_train(bes_ctx *c, int steps, double lr, int nprobe, int nrich, unsign
   im->lp_current = false;
   BES_HIP(hipStreamSynchronize(c->stream));
   return prof_collect(c);
+}
+
+int bes_lp_set_online(bes_ctx *c, int steps, double lr, int nprobe, int nrich) {
+  Impl *im = impl_get(c);
+  if (!im || !im->lp || steps < 0 || !(lr > 0) || nprobe < 1 || nprobe > 16 || nrich < 1 || nrich > 8) {
+    set_error("bes_lp_init first / steps >= 0, lr > 0, probes 1-16, Richardson steps 1-8");
+    return BES_ERR_ARG;
+  }
+  im->lp_online_steps = steps;
+  im->lp_online_lr = lr;
+  im->lp_online_nprobe = nprobe;
+  im->lp_online_nrich = nrich;
+  return BES_OK;
+}
+
+double bes_lp_online_loss(bes_ctx *c) {
+  Impl *im = impl_get(c);
+  return im ? im->lp_online_loss : 0.0;
 }
 
 int bes_lp_fallbacks(bes_ctx *c) {

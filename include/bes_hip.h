@@ -85,11 +85,15 @@ int bes_set_freestream(bes_ctx *ctx, const double *primitive);
 /* ---- batched conditions (no reference counterpart: the reference runs one
  *      freestream condition per process, Config.cpp:479-488).  nbatch, a power
  *      of two in [1, 64], freestream conditions share the mesh and advance
- *      together; every per-cell host array of the state, time-step, gradient,
- *      limiter and residual calls below then holds nbatch blocks, condition-major
+ *      together, one launch per kernel for all of them; every per-cell host
+ *      array of the state, time-step, gradient, limiter, residual and linear-
+ *      solution calls below then holds nbatch blocks, condition-major
  *      ([nbatch][ncell][...]), and per-variable outputs hold [nbatch][nv].
- *      Changing the batch size discards the device state.  The implicit path
- *      requires nbatch = 1. ---- */
+ *      Changing the batch size discards the device state and the implicit
+ *      workspace (call bes_implicit_setup again).  The implicit path takes a
+ *      batch with GMRES and precond 0-2; each condition then runs its own
+ *      Arnoldi process, stopping point and pseudo-time loop, and its results
+ *      are the same bits as when it runs alone. ---- */
 int bes_set_batch(bes_ctx *ctx, int nbatch);
 int bes_get_batch(const bes_ctx *ctx);
 /* freestream primitive per condition, [nbatch][nv+1] */
@@ -180,6 +184,14 @@ int bes_implicit_stats(bes_ctx *ctx, long *reorths, int *lp_fallbacks, long *ext
  * LGMRES is not available: BES_ERR_ARG. */
 enum { BES_SOLVER_GMRES = 0, BES_SOLVER_BICGSTAB = 1 };
 int bes_implicit_set_solver(bes_ctx *ctx, int solver);
+/* order of the multicolour block ILU(0) (no reference counterpart: AMGCL's
+ * ILU0 is sequential): the colours come from a greedy pass over the cells in
+ * device (Morton) order (0, the default), the mesh file's cell order (1), or
+ * along the freestream velocity (2; uses the freestream set before).  Changes
+ * the factors and so the iteration counts, never the solution of a solve that
+ * converges.  bes_implicit_ncolors: colours of the current order. */
+int bes_implicit_set_coloring(bes_ctx *ctx, int mode);
+int bes_implicit_ncolors(bes_ctx *ctx);
 /* time step, gradient, limiter, face Jacobians and the system A x = b at the
  * current state (SolverImplicit.cpp:307-320 = resetSystemOfEquation_ + dt +
  * gradient + limiter + computeFaceFluxes_).  repeat_nv != 0 reproduces the
@@ -197,11 +209,18 @@ int bes_implicit_get_system(bes_ctx *ctx, long *rowptr, long *col, double *val, 
  * CoefficientMatrix.cpp:166-204); val [nnz], rhs [ncell][nv] */
 int bes_implicit_set_system(bes_ctx *ctx, const double *val, const double *rhs);
 /* CoefficientMatrix::solve (CoefficientMatrix.cpp:210-325): iterations and the
- * true relative residual ||b - A x|| / ||b||; BES_ERR_SOLVE if rtol was not met */
+ * true relative residual ||b - A x|| / ||b||; BES_ERR_SOLVE if rtol was not met
+ * (for a batch: the largest of the conditions'; per condition see
+ * bes_implicit_batch_state) */
 int bes_linear_solve(bes_ctx *ctx, int *iters, double *relres);
 int bes_get_linear_solution(bes_ctx *ctx, double *x /*[ncell][nv]*/);
 /* U += x (SolverImplicit::updateSolution_, SolverImplicit.cpp:509-528) */
 int bes_implicit_update(bes_ctx *ctx);
+/* per condition of the batch (arrays [nbatch], any may be NULL): steps taken
+ * and drop reached in the loop of bes_implicit_run/_continue, the Courant
+ * number of its next step, Krylov iterations and true relative residual of
+ * the last solve (no reference counterpart) */
+int bes_implicit_batch_state(bes_ctx *ctx, int *steps, int *reached, double *cfl, int *iters, double *relres);
 /* SolverImplicit::solve (SolverImplicit.cpp:552-600): up to nsteps
  * backward-Euler steps with the adaptive Courant number of SolverImplicit.cpp:
  * 530-550 (cfl_min = COURANT_NUM, cfl_max = ADAPTIVE_COURANT_NUM_MAX,
@@ -211,7 +230,11 @@ int bes_implicit_update(bes_ctx *ctx);
  * var, Linf(sum F*A) per var, Courant number, Krylov iterations, relres. */
 int bes_implicit_run(bes_ctx *ctx, int nsteps, double cfl_min, double cfl_max, const double *exponents,
                      int local, int repeat_nv, double stop_drop, double *hist, int *steps_done);
-/* nsteps more steps of the loop the last bes_implicit_run started (same
+/* With a batch, hist is [nsteps][nbatch][2*nv+3], every condition keeps its
+ * own Courant number and stops (is frozen: no solve, no update) once its drop
+ * is reached; the rows of frozen conditions are NaN; linf_b and linf_r of
+ * bes_implicit_assemble are [nbatch][nv].
+ * nsteps more steps of the loop the last bes_implicit_run started (same
  * parameters; Courant number and residual history carry over), stopping early
  * at its stop_drop; reached = 1 once the drop has been reached. */
 int bes_implicit_continue(bes_ctx *ctx, int nsteps, double *hist, int *steps_done, int *reached);
@@ -246,6 +269,15 @@ int bes_lp_loss_grad(bes_ctx *ctx, const double *probes, int nprobe, int nrich, 
 int bes_lp_train(bes_ctx *ctx, int steps, double lr, int nprobe, int nrich, unsigned long long seed,
                  double *loss_hist);
 int bes_lp_fallbacks(bes_ctx *ctx);
+/* online training (no reference counterpart): with steps > 0, every solve of
+ * the learned form first runs that many Adam steps (learning rate lr, the
+ * right-hand side and nprobe - 1 seeded Gaussian probes, nrich Richardson
+ * steps) on the system the implicit loop has just assembled, so the network is
+ * trained on the Jacobians and residuals the solver itself generates; the
+ * operator is then rebuilt for the solve.  steps 0 switches it off.
+ * bes_lp_online_loss: the loss of the last online step. */
+int bes_lp_set_online(bes_ctx *ctx, int steps, double lr, int nprobe, int nrich);
+double bes_lp_online_loss(bes_ctx *ctx);
 /* the operator is rebuilt at every n-th solve after an assembly and reused by
  * the solves between (1: every solve); new parameters or training force a
  * rebuild.  bes_lp_builds: operator builds so far (no reference counterpart) */
@@ -307,6 +339,18 @@ int bes_write_slip_wall(bes_ctx *ctx, const char *path, int cond);
  *      ops, 8 factorisation, 9 implicit update, 10 face flux, 11 learned
  *      preconditioner (build and training) ---- */
 int bes_set_profiling(bes_ctx *ctx, int on);
+/* autotuner (no reference counterpart; the reference's counterpart is its
+ * OpenMP schedule, SolverExplicit.cpp:285): times each tunable kernel kind at
+ * block sizes 64, 128 and 256 on the context's current data with HIP events
+ * and keeps the fastest.  flags: 1 the explicit kernels (time step, face flux,
+ * residual) on the current state; 2 the implicit kernels (Jacobians, assembly,
+ * factorisation, SpMV, preconditioner, update) on the assembled system, which
+ * is assembled and factored again (same values); 4 the ILU(0) colouring order
+ * (bes_implicit_set_coloring), chosen as the one whose solve of the assembled
+ * system is fastest (the linear solution is overwritten).  chosen
+ * [16] (block size per kernel kind, as bes_get_profile numbers them), best_us
+ * [16] (their time, 0 when not tuned) and coloring may be NULL. */
+int bes_autotune(bes_ctx *ctx, int flags, int *chosen, double *best_us, int *coloring);
 int bes_get_profile(bes_ctx *ctx, double *total_ms, long *launches, double *bytes, int nkinds);
 int bes_synchronize(bes_ctx *ctx);
 

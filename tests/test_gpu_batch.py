@@ -111,10 +111,16 @@ def test_batch_argument_checks():
                 c.set_batch(bad)
             assert ei.value.status == core.ERR_ARG
         c.set_batch(2)
-        with pytest.raises(core.BesError) as ei:
-            c.implicit_setup()
+        with pytest.raises(core.BesError) as ei:   # a batch runs precond 0-2
+            c.implicit_setup(precond="learned")
         assert ei.value.status == core.ERR_ARG
-        c.set_batch(1)
+        c.implicit_setup(precond="ilu0")
+        c.set_batch(4)                             # a new batch size releases the implicit workspace
+        with pytest.raises(core.BesError) as ei:
+            c.implicit_assemble(1.0)
+        assert ei.value.status == core.ERR_ARG
+        P = core.freestream_primitive((0.5, 0.1), 2)
+        c.set_freestream(P)
+        c.set_state(np.tile(core.conserved_from_primitive(P, 2), (4, md.ncell, 1)))
         c.implicit_setup(precond="jacobi")
-        with pytest.raises(core.BesError):
-            c.set_batch(4)
+        assert c.implicit_assemble(1.0)[0].shape == (4, 4)

@@ -132,8 +132,16 @@ def test_fp32_inner_solve():
         A2 = A.tocsr()
         A2.sort_indices()
         assert np.array_equal(Ag.indices, A2.indices) and np.array_equal(Ag.indptr, A2.indptr)
-        # FP32 storage of the FP64 assembly: relative rounding of each entry
-        assert (np.abs(Ag.data - A2.data) <= 6e-8 * np.abs(A2.data)).all()
+        # FP32 assembly: the face Jacobians are stored in FP32 before the block
+        # sums and the blocks are stored in FP32, so each nv x nv block agrees
+        # with the FP64 assembly to a few FP32 roundings of its largest entry
+        # (a diagonal block sums the cell's face terms, which can cancel)
+        nv = 5
+        bg = Ag.toarray().reshape(-1, nv, A.shape[0] // nv, nv).transpose(0, 2, 1, 3)
+        b64 = A2.toarray().reshape(-1, nv, A.shape[0] // nv, nv).transpose(0, 2, 1, 3)
+        bmax = np.abs(b64).max(axis=(2, 3), keepdims=True)
+        assert (np.abs(bg - b64) <= 3e-7 * bmax).all()
+        assert (np.abs(bg - b64) > 0).any()    # really rounded
         assert np.array_equal(rhs_g, rhs)      # right-hand side stays FP64
         it, rr = c.linear_solve()
         x = c.linear_solution()

@@ -1,0 +1,120 @@
+"""Pins of the oracle's loop-level formulas by what the mathematics fixes.
+
+The oracle's residual, time step, gradient, limiter and flux Jacobians equal
+the compiled reference bitwise (test_oracle_golden).  Three formulas of the
+reference's solver loops could not be compiled (Solver*.cpp needs AMGCL, hence
+Boost) and are restated in both the oracle and the golden driver; these tests
+pin them independently:
+
+* the implicit right-hand side is b = -(dt/V) R, R the second-order residual
+  (SolverImplicit.cpp:441-473);
+* the assembled matrix is A = I + r (dt/V) dR1/dU, R1 the first-order residual
+  and r = nV with the reference's repeated block adds (Q4) or 1 without
+  (SolverImplicit.cpp:418-502), checked by central finite differences of R1;
+* the explicit multistage step is U_k = U_0 - alpha_k dt/V R(U_(k-1)) with dt
+  from the step's start state (SolverExplicit.cpp:259-393), composed here from
+  the pinned pieces and compared with the compiled reference's own iterations.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from conftest import golden_mesh
+from oracle import oracle as orc
+
+
+def _mesh_and_finf(g):
+    nd = int(g["ndim"])
+    return orc.Mesh(golden_mesh(g)), orc.freestream_primitive(g["mach"], nd), nd
+
+
+def test_rhs_is_scaled_residual(golden):
+    name, g = golden
+    scale = (g["dt_local"] / g["cell_vol"])[:, None]
+    err = np.abs(g["impl_rhs"] + scale * g["resid"])
+    # each face's term is rounded once more in the reference's order (dt/V*A)*F
+    bound = 4e-16 * scale * g["resid_scale"] + 1e-300
+    assert (err <= bound).all(), (name, (err / bound).max())
+    assert np.abs(g["impl_rhs"]).max() > 0
+
+
+def _first_order_residual(om, U, finf):
+    z = np.zeros((om.ncell, om.nv, om.ndim))
+    return om.residual(U, z, np.zeros((om.ncell, om.nv)), finf)
+
+
+@pytest.mark.parametrize("repeat", [True, False])
+def test_assembled_operator_is_first_order_jacobian(golden, repeat):
+    name, g = golden
+    om, finf, nd = _mesh_and_finf(g)
+    nv = nd + 2
+    U = np.ascontiguousarray(g["state"])
+    cfl = float(g["cfl"])
+    rp, col, val, rhs, dt = om.implicit_assemble(U, cfl, finf, repeat_nv=repeat)
+    A = sp.bsr_matrix((val, col, rp), shape=(om.ncell * nv,) * 2).tocsr()
+    rng = np.random.default_rng(5)
+    scale = (dt / om.cells()["vol"])[:, None] * (nv if repeat else 1)
+    for _ in range(3):
+        d = rng.normal(size=U.shape) * np.abs(U).max(0)
+        eps = 1e-6
+        dR = (_first_order_residual(om, U + eps * d, finf) - _first_order_residual(om, U - eps * d, finf)) / (2 * eps)
+        lhs = (A @ d.ravel()).reshape(U.shape) - d
+        rhs_fd = scale * dR
+        assert np.linalg.norm(lhs - rhs_fd) <= 2e-6 * np.linalg.norm(rhs_fd), name
+
+
+def test_operator_check_catches_a_transposed_block(golden):
+    """The finite-difference check above fails for a plausible assembly slip:
+    the off-diagonal blocks transposed."""
+    name, g = golden
+    om, finf, nd = _mesh_and_finf(g)
+    nv = nd + 2
+    U = np.ascontiguousarray(g["state"])
+    rp, col, val, rhs, dt = om.implicit_assemble(U, float(g["cfl"]), finf, repeat_nv=False)
+    bad = val.copy()
+    for e in range(om.ncell):
+        for b in range(rp[e] + 1, rp[e + 1]):
+            bad[b] = bad[b].T
+    A = sp.bsr_matrix((bad, col, rp), shape=(om.ncell * nv,) * 2).tocsr()
+    d = np.random.default_rng(6).normal(size=U.shape) * np.abs(U).max(0)
+    eps = 1e-6
+    dR = (_first_order_residual(om, U + eps * d, finf) - _first_order_residual(om, U - eps * d, finf)) / (2 * eps)
+    rhs_fd = (dt / om.cells()["vol"])[:, None] * dR
+    lhs = (A @ d.ravel()).reshape(U.shape) - d
+    assert np.linalg.norm(lhs - rhs_fd) > 1e-3 * np.linalg.norm(rhs_fd), name
+
+
+def _stage_composition(om, U0, cfl, finf, alpha):
+    """SolverExplicit::timeStep_ from its pieces: dt once from U0, then for each
+    stage the gradient, limiter and residual of the previous stage's state."""
+    dt = om.timestep(U0, cfl)
+    V = om.cells()["vol"]
+    Uk = U0
+    for a in alpha:
+        G = om.gradient(Uk)
+        P = om.limiter(Uk, G)
+        R = om.residual(Uk, G, P, finf)
+        Uk = U0 - (a * dt / V)[:, None] * R
+    return Uk
+
+
+@pytest.mark.parametrize("alpha", [(0.3, 1.0), orc.RK4_ALPHA])
+def test_multistage_step_is_the_stage_composition(golden, alpha):
+    name, g = golden
+    om, finf, nd = _mesh_and_finf(g)
+    U0 = np.ascontiguousarray(g["state"])
+    cfl = float(g["cfl"])
+    Uc = _stage_composition(om, U0, cfl, finf, alpha)
+    Uo, _ = om.explicit(U0, 1, cfl, finf, alpha=alpha, nthreads=1)
+    assert np.array_equal(Uc, Uo), name
+
+
+def test_reference_iterations_are_the_stage_composition(golden):
+    """The compiled reference units' explicit iterations (golden) equal the
+    composition of the pinned pieces, iteration after iteration."""
+    name, g = golden
+    om, finf, nd = _mesh_and_finf(g)
+    U = np.ascontiguousarray(g["state"])
+    for _ in range(int(g["niter"])):
+        U = _stage_composition(om, U, float(g["cfl"]), finf, orc.RK4_ALPHA)
+    assert np.array_equal(U, g["explicit_state"]), name
