@@ -579,28 +579,50 @@ __device__ __forceinline__ void block_store_sums(const double *acc, int nq, doub
 // condition fastest).  Every condition's n1 elements are summed in one fixed
 // tree whatever B is, so a condition's dot products, norms and hence its whole
 // Krylov solve are the same bits in a batch as alone: lane l of L (L depends
-// on n1 only) adds, in order, the element pairs (2(l + kL), 2(l + kL) + 1),
-// k = 0, 1, ...; the L lane sums of a condition are then added by
-// k_sum_lanes in a fixed order.  Thread l*B + b is lane l of condition b, so
-// the loads of a wave are contiguous for any B (packets when B = 1).
+// on n1 only) adds, in order, the elements of the groups g = l, l + L, l + 2L,
+// ..., group g being elements 4g .. 4g + 3 (zeros past n1); the L lane sums of
+// a condition are then added by sum_lanes in a fixed two-level tree.  Thread
+// l*B + b is lane l of condition b, so the loads of a wave are contiguous for
+// any B, and one 16-byte packet (FP32) or two (FP64) per vector when B = 1.
+constexpr int kLaneGroup = 4;
+
 template <typename TV>
-__device__ __forceinline__ void ld_pair(const TV *__restrict__ v, size_t e, size_t n1, int lgB, int b, double &x0,
-                                        double &x1) {
-  if (lgB == 0 && sizeof(TV) == 8 && e + 1 < n1) {
-    const double2 t = *reinterpret_cast<const double2 *>(v + e);
-    x0 = t.x;
-    x1 = t.y;
-  } else if (lgB == 0 && sizeof(TV) == 4 && e + 1 < n1) {
-    const float2 t = *reinterpret_cast<const float2 *>(v + e);
-    x0 = t.x;
-    x1 = t.y;
+__device__ __forceinline__ void ld_group(const TV *__restrict__ v, size_t e, size_t n1, int lgB, int b,
+                                         double x[kLaneGroup]) {
+  if (lgB == 0 && e + kLaneGroup <= n1) {
+    if (sizeof(TV) == 8) {
+      const double2 a = *reinterpret_cast<const double2 *>(v + e);
+      const double2 c = *reinterpret_cast<const double2 *>(v + e + 2);
+      x[0] = a.x; x[1] = a.y; x[2] = c.x; x[3] = c.y;
+    } else {
+      const float4 a = *reinterpret_cast<const float4 *>(v + e);
+      x[0] = a.x; x[1] = a.y; x[2] = a.z; x[3] = a.w;
+    }
   } else {
-    x0 = (double)v[(e << lgB) + b];
-    x1 = e + 1 < n1 ? (double)v[((e + 1) << lgB) + b] : 0.0;
+#pragma unroll
+    for (int h = 0; h < kLaneGroup; h++) x[h] = e + h < n1 ? (double)v[((e + h) << lgB) + b] : 0.0;
   }
 }
 
-// up to 8 dot products <V_(i0+q), w> and ||w||^2 per lane: S[(q*B + b)*L + l], q = 8 the norm
+template <typename TV>
+__device__ __forceinline__ void st_group(TV *__restrict__ v, size_t e, size_t n1, int lgB, int b,
+                                         const TV r[kLaneGroup]) {
+  if (lgB == 0 && e + kLaneGroup <= n1) {
+    if (sizeof(TV) == 8) {
+      *reinterpret_cast<double2 *>(v + e) = make_double2((double)r[0], (double)r[1]);
+      *reinterpret_cast<double2 *>(v + e + 2) = make_double2((double)r[2], (double)r[3]);
+    } else {
+      *reinterpret_cast<float4 *>(v + e) = make_float4((float)r[0], (float)r[1], (float)r[2], (float)r[3]);
+    }
+  } else {
+#pragma unroll
+    for (int h = 0; h < kLaneGroup; h++)
+      if (e + h < n1) v[((e + h) << lgB) + b] = r[h];
+  }
+}
+
+// up to 8 dot products <V_(i0+q), w> and ||w||^2 per lane: S[(q*B + b)*L + l],
+// the norm in slot cnt
 template <typename TV>
 __global__ __launch_bounds__(256) void k_mdot(const TV *__restrict__ V, size_t stride, int i0, int cnt,
                                               const TV *__restrict__ w, size_t n1, int lgB, int L,
@@ -613,28 +635,56 @@ __global__ __launch_bounds__(256) void k_mdot(const TV *__restrict__ V, size_t s
   double acc[9];
 #pragma unroll
   for (int q = 0; q < 9; q++) acc[q] = 0;
-  for (size_t e = 2 * l; e < n1; e += 2 * (size_t)L) {
-    double w0, w1;
-    ld_pair(w, e, n1, lgB, b, w0, w1);
+  for (size_t e = kLaneGroup * l; e < n1; e += (size_t)kLaneGroup * L) {
+    double wx[kLaneGroup];
+    ld_group(w, e, n1, lgB, b, wx);
 #pragma unroll
     for (int q = 0; q < 8; q++)
       if (q < cnt) {
-        double v0, v1;
-        ld_pair(V + (size_t)(i0 + q) * stride, e, n1, lgB, b, v0, v1);
-        acc[q] += v0 * w0;
-        acc[q] += v1 * w1;
+        double vx[kLaneGroup];
+        ld_group(V + (size_t)(i0 + q) * stride, e, n1, lgB, b, vx);
+#pragma unroll
+        for (int h = 0; h < kLaneGroup; h++) acc[q] += vx[h] * wx[h];
       }
-    acc[8] += w0 * w0;
-    acc[8] += w1 * w1;
+#pragma unroll
+    for (int h = 0; h < kLaneGroup; h++) acc[8] += wx[h] * wx[h];
   }
   for (int q = 0; q < cnt; q++) S[((size_t)q * B + b) * L + l] = acc[q];
-  S[((size_t)8 * B + b) * L + l] = acc[8];
+  S[((size_t)cnt * B + b) * L + l] = acc[8];
 }
 
-// out[qb] = sum over l of S[qb*L + l], one block per (q, condition) qb: thread t
-// adds lanes t, t + 256, ... in order, then a fixed tree over the 256 threads
+// level 1 of the lane sums: block (x, qb) adds lanes x*1024 + t + 256k of row
+// qb (k = 0..3 in order, thread t), then a fixed tree over the block.  With
+// one block per row (L <= 1024) the result is final and goes to
+// out[q'*out_stride_q + b] (qb = q*B + b; q' = 8 for q = qnorm, else q);
+// otherwise to S2[qb*G + x] for level 2.
+__global__ __launch_bounds__(256) void k_sum_lanes1(const double *__restrict__ S, int L, double *__restrict__ dst,
+                                                    int out_stride_q, int qnorm, int lgB) {
+  const int x = blockIdx.x, qb = blockIdx.y, G = gridDim.x, B = 1 << lgB;
+  double s = 0;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const int l = x * 1024 + k * 256 + (int)threadIdx.x;
+    if (l < L) s += S[(size_t)qb * L + l];
+  }
+  __shared__ double sh[4];
+  s = wave_sum(s);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  const double r = sh[0] + sh[1] + sh[2] + sh[3];
+  if (G > 1) {
+    dst[(size_t)qb * G + x] = r;
+  } else {
+    const int q = qb >> lgB, qo = q == qnorm ? 8 : q;
+    dst[(size_t)qo * out_stride_q + (qb & (B - 1))] = r;
+  }
+}
+
+// level 2: out row qb = sum of its G level-1 sums, thread t adding t, t + 256,
+// ... in order, then a fixed tree (same output mapping as level 1)
 __global__ __launch_bounds__(256) void k_sum_lanes(const double *__restrict__ S, int L, double *__restrict__ out,
-                                                   int out_stride_q, int lgB) {
+                                                   int out_stride_q, int qnorm, int lgB) {
   const int qb = blockIdx.x, B = 1 << lgB;
   double s = 0;
   for (int l = threadIdx.x; l < L; l += 256) s += S[(size_t)qb * L + l];
@@ -642,7 +692,10 @@ __global__ __launch_bounds__(256) void k_sum_lanes(const double *__restrict__ S,
   s = wave_sum(s);
   if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
   __syncthreads();
-  if (threadIdx.x == 0) out[(qb >> lgB) * out_stride_q + (qb & (B - 1))] = sh[0] + sh[1] + sh[2] + sh[3];
+  if (threadIdx.x == 0) {
+    const int q = qb >> lgB, qo = q == qnorm ? 8 : q;
+    out[(size_t)qo * out_stride_q + (qb & (B - 1))] = sh[0] + sh[1] + sh[2] + sh[3];
+  }
 }
 
 // w -= sum_q h_q V_q per condition (h_q of condition b at h[(9*(q/8) + q%8)*B + b],
@@ -663,25 +716,34 @@ __global__ __launch_bounds__(256) void k_orth_update(const TV *__restrict__ V, s
   const size_t l = g >> lgB;
   const int b = (int)(g & (B - 1));
   double nrm = 0;
-  for (size_t e = 2 * l; e < n1; e += 2 * (size_t)L) {
-    double w0, w1;
-    ld_pair(w, e, n1, lgB, b, w0, w1);
-    for (int q = 0; q < cnt; q++) {
-      double v0, v1;
-      ld_pair(V + (size_t)q * stride, e, n1, lgB, b, v0, v1);
+  for (size_t e = kLaneGroup * l; e < n1; e += (size_t)kLaneGroup * L) {
+    double wx[kLaneGroup];
+    ld_group(w, e, n1, lgB, b, wx);
+    int q = 0;
+    for (; q + 4 <= cnt; q += 4) {   // four vectors' loads in flight at once
+      double vx[4][kLaneGroup];
+#pragma unroll
+      for (int u = 0; u < 4; u++) ld_group(V + (size_t)(q + u) * stride, e, n1, lgB, b, vx[u]);
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const double hq = hs[((q + u) << lgB) + b];
+#pragma unroll
+        for (int k = 0; k < kLaneGroup; k++) wx[k] -= hq * vx[u][k];
+      }
+    }
+    for (; q < cnt; q++) {
+      double vx[kLaneGroup];
+      ld_group(V + (size_t)q * stride, e, n1, lgB, b, vx);
       const double hq = hs[(q << lgB) + b];
-      w0 -= hq * v0;
-      w1 -= hq * v1;
+#pragma unroll
+      for (int k = 0; k < kLaneGroup; k++) wx[k] -= hq * vx[k];
     }
-    const TV r0 = (TV)w0, r1 = (TV)w1;
-    w[(e << lgB) + b] = r0;
-    nrm += (double)r0 * (double)r0;
-    if (e + 1 < n1) {
-      w[((e + 1) << lgB) + b] = r1;
-      nrm += (double)r1 * (double)r1;
-    } else {
-      nrm += 0.0 * 0.0;
-    }
+    TV r[kLaneGroup];
+#pragma unroll
+    for (int k = 0; k < kLaneGroup; k++) r[k] = (TV)wx[k];
+    st_group(w, e, n1, lgB, b, r);
+#pragma unroll
+    for (int k = 0; k < kLaneGroup; k++) nrm += (double)r[k] * (double)r[k];
   }
   S[(size_t)b * L + l] = nrm;
 }
@@ -696,38 +758,91 @@ __global__ __launch_bounds__(256) void k_resid(const double *__restrict__ bv, co
   const size_t l = g >> lgB;
   const int b = (int)(g & (B - 1));
   double nrm = 0;
-  for (size_t e = 2 * l; e < n1; e += 2 * (size_t)L) {
-    for (int h = 0; h < 2; h++) {
-      const size_t ee = e + h;
-      if (ee >= n1) { nrm += 0.0 * 0.0; continue; }
-      const size_t q = (ee << lgB) + b;
-      const double v = bv[q] - (double)y[q];
-      r[q] = (TV)v;
+  for (size_t e = kLaneGroup * l; e < n1; e += (size_t)kLaneGroup * L) {
+    double bx[kLaneGroup], yx[kLaneGroup];
+    ld_group(bv, e, n1, lgB, b, bx);
+    ld_group(y, e, n1, lgB, b, yx);
+    TV rr[kLaneGroup];
+#pragma unroll
+    for (int k = 0; k < kLaneGroup; k++) {
+      const double v = bx[k] - yx[k];
+      rr[k] = (TV)v;
       nrm += v * v;
     }
+    st_group(r, e, n1, lgB, b, rr);
   }
   S[(size_t)b * L + l] = nrm;
+}
+
+// Element-wise vector kernels.  A thread works on groups of 4 consecutive
+// elements (one 16-byte packet in FP32, two in FP64; every vector starts on a
+// 64-element boundary) in a grid-stride loop; the grid's thread count is a
+// multiple of 64, so element t of every group a thread touches belongs to the
+// same condition, (4*tid + t) mod B, and per-condition factors are read once.
+template <typename T>
+__device__ __forceinline__ void ld4(const T *p, double x[4]) {
+  if (sizeof(T) == 8) {
+    const double2 a = *reinterpret_cast<const double2 *>(p);
+    const double2 c = *reinterpret_cast<const double2 *>(p + 2);
+    x[0] = a.x; x[1] = a.y; x[2] = c.x; x[3] = c.y;
+  } else {
+    const float4 a = *reinterpret_cast<const float4 *>(p);
+    x[0] = a.x; x[1] = a.y; x[2] = a.z; x[3] = a.w;
+  }
+}
+template <typename T>
+__device__ __forceinline__ void st4(T *p, const double x[4]) {
+  if (sizeof(T) == 8) {
+    *reinterpret_cast<double2 *>(p) = make_double2(x[0], x[1]);
+    *reinterpret_cast<double2 *>(p + 2) = make_double2(x[2], x[3]);
+  } else {
+    *reinterpret_cast<float4 *>(p) = make_float4((float)x[0], (float)x[1], (float)x[2], (float)x[3]);
+  }
 }
 
 // w = w / sqrt(n2[b]) per condition when n2[b] > 0 (the norm read on the
 // device, so the launch need not wait for the host)
 template <typename TV>
 __global__ void k_scale_norm(TV *__restrict__ w, const double *__restrict__ n2, size_t n, int lgB) {
-  const size_t B = (size_t)1 << lgB;
-  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+  const size_t B = (size_t)1 << lgB, tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x,
+               nt = (size_t)gridDim.x * blockDim.x, ng = n / 4;
+  double a[4];
+  bool on[4];
+#pragma unroll
+  for (int t = 0; t < 4; t++) {
+    const double s2 = n2[(4 * tid + t) & (B - 1)];
+    on[t] = s2 > 0;
+    a[t] = on[t] ? 1.0 / sqrt(s2) : 1.0;
+  }
+  for (size_t g = tid; g < ng; g += nt) {
+    double x[4];
+    ld4(w + 4 * g, x);
+#pragma unroll
+    for (int t = 0; t < 4; t++) x[t] = on[t] ? (double)(TV)(a[t] * x[t]) : x[t];
+    st4(w + 4 * g, x);
+  }
+  for (size_t e = 4 * ng + tid; e < n; e += nt) {
     const double s2 = n2[e & (B - 1)];
-    if (!(s2 > 0)) continue;
-    const double a = 1.0 / sqrt(s2);
-    w[e] = (TV)(a * (double)w[e]);
+    if (s2 > 0) w[e] = (TV)((1.0 / sqrt(s2)) * (double)w[e]);
   }
 }
 
-// y = a_b x (scale/copy with conversion), a per condition
+// y = a_b x (scale/copy with conversion), a per condition; y may be x
 template <typename TA, typename TB>
-__global__ void k_scale_b(const TA *__restrict__ x, CoefB a, TB *__restrict__ y, size_t n, int lgB) {
-  const size_t B = (size_t)1 << lgB;
-  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
-    y[e] = (TB)(a.c[e & (B - 1)] * (double)x[e]);
+__global__ void k_scale_b(const TA *x, CoefB a, TB *y, size_t n, int lgB) {
+  const size_t B = (size_t)1 << lgB, tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x,
+               nt = (size_t)gridDim.x * blockDim.x, ng = n / 4;
+  double c[4];
+#pragma unroll
+  for (int t = 0; t < 4; t++) c[t] = a.c[(4 * tid + t) & (B - 1)];
+  for (size_t g = tid; g < ng; g += nt) {
+    double v[4];
+    ld4(x + 4 * g, v);
+#pragma unroll
+    for (int t = 0; t < 4; t++) v[t] = c[t] * v[t];
+    st4(y + 4 * g, v);
+  }
+  for (size_t e = 4 * ng + tid; e < n; e += nt) y[e] = (TB)(a.c[e & (B - 1)] * (double)x[e]);
 }
 
 // out[q] = sum over blocks of part[q][.], fixed order (deterministic)
@@ -743,33 +858,41 @@ __global__ void k_reduce_parts(const double *__restrict__ part, int nblk, int nq
   if (threadIdx.x == 0) out[q] = sh[0] + sh[1] + sh[2] + sh[3];
 }
 
-// y = a x for two vectors of one type, packets
-template <typename TV>
-__global__ void k_scale_p(const TV *__restrict__ x, double a, TV *__restrict__ y, size_t n) {
-  constexpr int W = Pk<TV>::W;
-  const size_t np = n / W, tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x, nt = (size_t)gridDim.x * blockDim.x;
-  for (size_t e = tid; e < np; e += nt) {
-    Pk<TV> v = ldp(x + e * W);
-#pragma unroll
-    for (int t = 0; t < W; t++) v.v[t] = (TV)(a * (double)v.v[t]);
-    stp(y + e * W, v);
-  }
-  for (size_t e = np * W + tid; e < n; e += nt) y[e] = (TV)(a * (double)x[e]);
-}
-
 // y = a*x (scale/copy with conversion)
 template <typename TA, typename TB>
 __global__ void k_scale(const TA *__restrict__ x, double a, TB *__restrict__ y, size_t n) {
-  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
-    y[e] = (TB)(a * (double)x[e]);
+  const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x, nt = (size_t)gridDim.x * blockDim.x,
+               ng = n / 4;
+  for (size_t g = tid; g < ng; g += nt) {
+    double v[4];
+    ld4(x + 4 * g, v);
+#pragma unroll
+    for (int t = 0; t < 4; t++) v[t] = a * v[t];
+    st4(y + 4 * g, v);
+  }
+  for (size_t e = 4 * ng + tid; e < n; e += nt) y[e] = (TB)(a * (double)x[e]);
 }
 
 // u = sum_q c[q][b] V[q] per condition
 template <typename TV>
 __global__ void k_combine(const TV *__restrict__ V, size_t stride, int cnt, const double *__restrict__ c,
                           TV *__restrict__ u, size_t n, int lgB) {
-  const size_t B = (size_t)1 << lgB;
-  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+  const size_t B = (size_t)1 << lgB, tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x,
+               nt = (size_t)gridDim.x * blockDim.x, ng = n / 4;
+  int bt[4];
+#pragma unroll
+  for (int t = 0; t < 4; t++) bt[t] = (int)((4 * tid + t) & (B - 1));
+  for (size_t g = tid; g < ng; g += nt) {
+    double s[4] = {0, 0, 0, 0};
+    for (int q = 0; q < cnt; q++) {
+      double v[4];
+      ld4(V + (size_t)q * stride + 4 * g, v);
+#pragma unroll
+      for (int t = 0; t < 4; t++) s[t] += c[q * B + bt[t]] * v[t];
+    }
+    st4(u + 4 * g, s);
+  }
+  for (size_t e = 4 * ng + tid; e < n; e += nt) {
     const size_t b = e & (B - 1);
     double s = 0;
     for (int q = 0; q < cnt; q++) s += c[q * B + b] * (double)V[(size_t)q * stride + e];
@@ -780,8 +903,17 @@ __global__ void k_combine(const TV *__restrict__ V, size_t stride, int cnt, cons
 // x (FP64) += z
 template <typename TV>
 __global__ void k_axpy_x(double *__restrict__ x, const TV *__restrict__ z, size_t n) {
-  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
-    x[e] += (double)z[e];
+  const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x, nt = (size_t)gridDim.x * blockDim.x,
+               ng = n / 4;
+  for (size_t g = tid; g < ng; g += nt) {
+    double xv[4], zv[4];
+    ld4(x + 4 * g, xv);
+    ld4(z + 4 * g, zv);
+#pragma unroll
+    for (int t = 0; t < 4; t++) xv[t] += zv[t];
+    st4(x + 4 * g, xv);
+  }
+  for (size_t e = 4 * ng + tid; e < n; e += nt) x[e] += (double)z[e];
 }
 
 // implicit update U[v][i][b] += x[i][v][b] for the conditions still advancing
@@ -999,14 +1131,16 @@ static int impl_alloc(bes_ctx *c, Impl *im) {
   BES_HIP(hipMalloc((void **)&im->xc, nvec * Bs * sizeof(double)));
   im->vstride = (nvec * Bs + 63) / 64 * 64;   // 16-byte aligned packet loads on every vector
   // lanes of the deterministic reductions: a power of two set by the size of
-  // one condition's vector only (each lane adds about 64 elements)
+  // one condition's vector only (each lane adds 16 to 64 elements; 2^18 lanes,
+  // a thread each, for the 2M-cell wing)
   im->lanes = 256;
-  while (im->lanes < 131072 && (size_t)im->lanes * 64 < nvec) im->lanes *= 2;
+  while (im->lanes < (1 << 20) && (size_t)im->lanes * 64 < nvec) im->lanes *= 2;
   // GMRES(m) keeps m + 1 basis vectors; BiCGStab's seven vectors use the same storage
   BES_HIP(hipMalloc(&im->V, (size_t)std::max(im->restart + 1, 7) * im->vstride * es));
   BES_HIP(hipMalloc(&im->Z, 3 * im->vstride * es));
   BES_HIP(hipMalloc((void **)&im->part, 32 * kRedBlocks * sizeof(double)));
   BES_HIP(hipMalloc((void **)&im->S, (size_t)9 * Bs * im->lanes * sizeof(double)));
+  BES_HIP(hipMalloc((void **)&im->S2, (size_t)9 * Bs * ((im->lanes + 1023) / 1024) * sizeof(double)));
   BES_HIP(hipMalloc((void **)&im->dots, 64 * Bs * sizeof(double)));
   BES_HIP(hipHostMalloc((void **)&im->hdots, 64 * Bs * sizeof(double), hipHostMallocDefault));
   BES_HIP(hipEventCreateWithFlags(&im->ev_dots, hipEventDisableTiming));
@@ -1017,7 +1151,7 @@ static int impl_alloc(bes_ctx *c, Impl *im) {
 static void impl_release(Impl *im) {
   lp::lp_release(im);
   amg_release(im);
-  void *ptrs[] = {im->fd.fo, im->fd.fn, im->fd.fnrm, im->fd.cf, im->fd.mrev, im->color, im->crow, im->JF, im->S,
+  void *ptrs[] = {im->fd.fo, im->fd.fn, im->fd.fnrm, im->fd.cf, im->fd.mrev, im->color, im->crow, im->JF, im->S, im->S2,
                   im->b, im->x, im->A, im->LU, im->LM, im->LMh, im->V, im->Z, im->part, im->dots, im->bits, im->Ac,
                   im->bc, im->xc,
                   im->cpos, im->cnb, im->cnp, im->clow, im->Dr, im->Ur};
@@ -1206,6 +1340,20 @@ static int spmv(bes_ctx *c, Impl *im, const TV *x, TV *y) {
   return BES_OK;
 }
 
+// the lane sums of rows qb = q*B + b < nqb of S (L lanes each) into
+// out[q'*out_stride_q + b], q' = 8 for q = qnorm, else q: level 1 over groups
+// of 1024 lanes, then level 2 over the groups (skipped for L <= 1024)
+static void sum_lanes(bes_ctx *c, Impl *im, const double *S, int nqb, double *out, int out_stride_q, int qnorm) {
+  const int L = im->lanes, G = (L + 1023) / 1024;
+  if (G == 1) {
+    hipLaunchKernelGGL(k_sum_lanes1, dim3(1, nqb), dim3(256), 0, c->stream, S, L, out, out_stride_q, qnorm, c->lgB);
+  } else {
+    hipLaunchKernelGGL(k_sum_lanes1, dim3(G, nqb), dim3(256), 0, c->stream, S, L, im->S2, out_stride_q, qnorm,
+                       c->lgB);
+    hipLaunchKernelGGL(k_sum_lanes, dim3(nqb), dim3(256), 0, c->stream, im->S2, G, out, out_stride_q, qnorm, c->lgB);
+  }
+}
+
 // dots h_q = V_q^T w (q < cnt) and ||w||^2 of every condition: chunk ch of up to
 // 8 vectors reduces into dots[(9*ch + q)*B + b], its ||w||^2 into
 // dots[(9*ch + 8)*B + b] (the deterministic lane reductions)
@@ -1220,11 +1368,7 @@ static int multidot(bes_ctx *c, Impl *im, const TV *V, size_t stride, int cnt, c
   for (int q = 0; q < nchunk; q++) {
     const int i0 = q * 8, cc = std::max(0, std::min(8, cnt - i0));
     hipLaunchKernelGGL((k_mdot<TV>), dim3(nb), dim3(256), 0, c->stream, V, stride, i0, cc, w, n1, lgB, L, im->S);
-    if (cc > 0)
-      hipLaunchKernelGGL(k_sum_lanes, dim3(cc * B), dim3(256), 0, c->stream, im->S, L, im->dots + (size_t)9 * q * B,
-                         B, lgB);
-    hipLaunchKernelGGL(k_sum_lanes, dim3(B), dim3(256), 0, c->stream, im->S + (size_t)8 * B * L, L,
-                       im->dots + (size_t)(9 * q + 8) * B, B, lgB);
+    sum_lanes(c, im, im->S, (cc + 1) * B, im->dots + (size_t)9 * q * B, B, cc);
   }
   BES_HIP(hipGetLastError());
   return BES_OK;
@@ -1232,8 +1376,7 @@ static int multidot(bes_ctx *c, Impl *im, const TV *V, size_t stride, int cnt, c
 
 // one lane-reduced norm per condition (S slot 0) into dots[at*B + b]
 static int sum_norms(bes_ctx *c, Impl *im, int at) {
-  hipLaunchKernelGGL(k_sum_lanes, dim3(c->B), dim3(256), 0, c->stream, im->S, im->lanes, im->dots + (size_t)at * c->B,
-                     c->B, c->lgB);
+  sum_lanes(c, im, im->S, c->B, im->dots + (size_t)at * c->B, c->B, -1);
   BES_HIP(hipGetLastError());
   return BES_OK;
 }

@@ -72,6 +72,7 @@ struct Impl {
   double *part = nullptr;  // reduction partials [32][kRedBlocks] (BiCGStab)
   int lanes = 0;           // lanes L of the deterministic batched reductions
   double *S = nullptr;     // their lane sums [9][B][L]
+  double *S2 = nullptr;    // level-1 sums of groups of 1024 lanes [9][B][L/1024]
   double *dots = nullptr;  // [64][B] reduced dot products, condition fastest
   double *hdots = nullptr; // [64][B] pinned host copy of dots (Arnoldi step)
   hipEvent_t ev_dots = nullptr;  // recorded after the copy into hdots

@@ -190,3 +190,65 @@ def test_lagged_operator_rebuild():
         c.implicit_assemble(13.0)
         c.linear_solve()
         assert c.lp_builds() == 3
+
+
+def _solve_counts(md, P, U, params, cfls, precision=32):
+    """GMRES(30) iterations of the learned form (rebuilt every solve, no
+    fallback needed at rtol 5e-2) on the systems of a fixed state at a list of
+    Courant numbers."""
+    with core.Context(md) as c:
+        c.set_freestream(P)
+        c.set_state(U)
+        c.implicit_setup(precision, "learned", restart=30, maxiter=300, rtol=5e-2)
+        c.lp_init(20261202)
+        c.lp_set_params(params)
+        its = []
+        for cfl in cfls:
+            c.implicit_assemble(cfl, repeat_nv=False)
+            its.append(c.linear_solve(allow_unconverged=True)[0])
+        return its, c.lp_fallbacks()
+
+
+def test_online_training_beats_untrained():
+    """The network starts neutral (every blend 1/2, no prior) and is trained on
+    the GPU, a few Adam steps on each system the implicit loop assembles on a
+    9k-tet wing at M 0.84; on held-out systems (another flow condition, states
+    along its own run) the trained operator needs fewer GMRES iterations than
+    the untrained one."""
+    md = mg.swept_wing(16, 8, 12)
+    P = core.freestream_primitive(core.wing_freestream(0.84, 3.06), 3)
+    U0 = np.tile(core.conserved_from_primitive(P, 3), (md.ncell, 1))
+    with core.Context(md) as c:
+        c.set_freestream(P)
+        c.set_state(U0)
+        c.implicit_setup(32, "learned", restart=30, maxiter=300, rtol=5e-2)
+        c.lp_init(20261202)
+        c.lp_neutral()
+        neutral = c.lp_params()
+        c.lp_set_online(2, lr=3e-3, nprobe=4, nrich=3)
+        h = c.implicit_run(40, 1.0, 1000.0, (0.5, 1.5), repeat_nv=False)
+        trained = c.lp_params()
+        print("training run: steps", len(h), "last loss", c.lp_online_loss(), flush=True)
+    assert not np.array_equal(trained, neutral)
+    # held-out: a different condition; states after 10, 25 and 40 of its own steps (ILU(0) run)
+    P2 = core.freestream_primitive(core.wing_freestream(0.75, 1.5), 3)
+    states = []
+    with core.Context(md) as c:
+        c.set_freestream(P2)
+        c.set_state(np.tile(core.conserved_from_primitive(P2, 3), (md.ncell, 1)))
+        c.implicit_setup(64, "ilu0", rtol=5e-2, maxiter=300)
+        for n in (10, 15, 15):
+            if not states:
+                c.implicit_run(n, 1.0, 1000.0, (0.5, 1.5), repeat_nv=False)
+            else:
+                c.implicit_continue(n)
+            states.append(c.get_state())
+    cfls = (50.0, 300.0, 1000.0)
+    tot = {"neutral": 0, "trained": 0}
+    for U in states:
+        for name, p in (("neutral", neutral), ("trained", trained)):
+            its, fb = _solve_counts(md, P2, U, p, cfls)
+            tot[name] += sum(its)
+            print(name, its, "fallbacks", fb, flush=True)
+    print(tot)
+    assert tot["trained"] < tot["neutral"]
