@@ -125,22 +125,32 @@ def implicit_workload(args):
         geo = c.geometry()
         log(f"context ready at {time.perf_counter() - t0:.1f}s")
 
-        def setup(precision, precond):
-            c.implicit_setup(precision, precond, restart=args.restart, maxiter=args.krylov_max, rtol=args.rtol)
+        tuned = {"coloring": None}
+
+        def setup(precision, precond, params=None):
+            kind = "learned" if precond.startswith("learned") else precond
+            c.implicit_setup(precision, kind, restart=args.restart, maxiter=args.krylov_max, rtol=args.rtol)
             c.implicit_set_reorth(args.reorth)
-            if precond == "learned":
+            if kind in ("ilu0", "learned") and tuned["coloring"]:   # the ILU(0) fallback of the learned form too
+                c.implicit_set_coloring(tuned["coloring"])
+            if kind == "learned":
                 c.lp_init(20261202)
+                if precond == "learned":
+                    c.lp_set_params(params)
+                elif precond == "learned_untrained":
+                    c.lp_neutral()
                 c.lp_set_rebuild_interval(args.lp_rebuild)
             if precond == "amg":
                 c.amg_set_params(relax="block_jacobi", rebuild_interval=args.amg_rebuild)
+            c.set_limiter_freeze(args.limiter_freeze)   # 0: the reference's limiter at every pass
 
         def forces():
             U = c.get_state()
             return mg.force_coefficients(geo["S"], _centers(md, geo), geo["btype"], geo["owner"], U, mach,
                                          alpha, spec)[:2]
 
-        def drop_run(name, precision, precond):
-            setup(precision, precond)
+        def drop_run(name, precision, precond, params=None):
+            setup(precision, precond, params)
             c.set_state(U0)
             c.synchronize()
             t = time.perf_counter()
@@ -184,6 +194,16 @@ def implicit_workload(args):
             ramp.append(h)
         R = np.concatenate(ramp)
         log(f"ramp: {nramp} steps to CFL {R[nramp - 1, 10]:.1f}, then {args.warmup} warmup steps")
+        if args.autotune:
+            # block sizes and the ILU(0) colouring, timed on this mesh's system at the CFL cap
+            ta = time.perf_counter()
+            c.implicit_assemble(R[-1, 10], repeat_nv=repeat)
+            bs, col = c.autotune(implicit=True, coloring=tprecond == "ilu0")
+            tuned["coloring"] = col
+            out["autotune"] = {"block_size_us": bs, "ilu0_coloring": col,
+                               "ncolors": c.implicit_ncolors() if tprecond == "ilu0" else None,
+                               "wall_s": time.perf_counter() - ta}
+            log(f"autotune {time.perf_counter() - ta:.1f}s: {bs}, coloring {col}")
         # state, Courant number and monitors at the start of the timed steps
         # (the CPU baseline continues from exactly this point)
         U_start = c.get_state()
@@ -223,7 +243,15 @@ def implicit_workload(args):
             if "ilu0" in forms and not args.quick_drop:
                 variants["mixed_fp32_ilu0"] = drop_run("mixed_fp32_ilu0", 32, "ilu0")
             if "learned" in forms:
-                variants["mixed_fp32_learned"] = drop_run("mixed_fp32_learned", 32, "learned")
+                params, training = train_learned(args, md)
+                out["learned_training"] = training
+                variants["mixed_fp32_learned"] = drop_run("mixed_fp32_learned", 32, "learned", params)
+            if "learned_untrained" in forms:
+                variants["mixed_fp32_learned_untrained"] = drop_run("mixed_fp32_learned_untrained", 32,
+                                                                    "learned_untrained")
+            if "learned_handset" in forms:
+                variants["mixed_fp32_learned_handset"] = drop_run("mixed_fp32_learned_handset", 32,
+                                                                  "learned_handset")
             if "amg" in forms:
                 variants["mixed_fp32_amg"] = drop_run("mixed_fp32_amg", 32, "amg")
             plain["cell_updates_per_s_whole_run"] = md.ncell * plain["steps"] / plain["wall_s"]
@@ -257,6 +285,9 @@ def implicit_workload(args):
                          "FP32 matrix, preconditioner and Krylov solve under the FP64 residual",
             "timed": f"{args.steps} steps of the {args.timed_form} form at the CFL cap: the ramp ran "
                      f"{nramp} untimed steps until a step used CFL {cmax:g}, then {args.warmup} untimed steps"}
+    if args.limiter_freeze > 0:
+        cfgd["limiter_freeze"] = (f"each form fixes the limiter once its density residual drop reaches "
+                                  f"{args.limiter_freeze:g} (no reference counterpart)")
     res = {"metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps, "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "f64" if tprec == 64 else "f64 residual, f32 solve",
@@ -269,8 +300,39 @@ def implicit_workload(args):
     return res
 
 
-LEARNED_NOTE = ("network with its initial weights (blend 0.953 between block Jacobi and the first-order "
-                "Neumann operator), rebuilt every %d solves")
+LEARNED_NOTE = ("network trained on the GPU from a neutral start (every blend 1/2) by online Adam steps on the "
+                "systems of an implicit run on the 0.5M-tet wing (learned_training), then used unchanged on the "
+                "2M wing, rebuilt every %d solves; _untrained: the neutral network; _handset: the hand-set "
+                "decoder bias (blend 0.953)")
+
+
+def train_learned(args, md_eval):
+    """Train the learned preconditioner on the GPU, from a neutral start, on
+    the Jacobians and residuals an implicit run generates on the 0.5M-tet wing
+    (a mesh the timed 2M runs never see): a few Adam steps on each assembled
+    system (bes_lp_set_online)."""
+    from bes_amd import core, meshgen as mg
+    md = mg.swept_wing(*args.sweep_wing)
+    finf = core.freestream_primitive(core.wing_freestream(0.84, 3.06), 3)
+    cmin, cmax, ex = WING_CFL
+    t0 = time.perf_counter()
+    with core.Context(md) as c:
+        c.set_freestream(finf)
+        c.set_state(np.tile(core.conserved_from_primitive(finf, 3), (md.ncell, 1)))
+        c.implicit_setup(32, "learned", restart=args.restart, maxiter=args.krylov_max, rtol=args.rtol)
+        c.implicit_set_reorth(args.reorth)
+        c.lp_init(20261202)
+        c.lp_neutral()
+        c.lp_set_online(args.lp_train_steps, lr=args.lp_lr, nprobe=4, nrich=3)
+        h = c.implicit_run(args.lp_train_pseudo, cmin, cmax, ex, stop_drop=1e-6, repeat_nv=False)
+        params = c.lp_params()
+        loss = c.lp_online_loss()
+    wall = time.perf_counter() - t0
+    log(f"learned training: {len(h)} pseudo-time steps x {args.lp_train_steps} Adam steps on {md.ncell} tets, "
+        f"{wall:.1f}s, last loss {loss:.3f}")
+    return params, {"mesh_cells": md.ncell, "pseudo_time_steps": len(h), "adam_steps_per_system":
+                    args.lp_train_steps, "systems": len(h), "learning_rate": args.lp_lr, "last_loss": loss,
+                    "wall_s": wall, "krylov_per_step_first_last": [int(h[0, 11]), int(h[-1, 11])]}
 
 
 def _centers(md, geo):
@@ -635,6 +697,8 @@ def main():
     ap.add_argument("--wing", type=int, nargs=3, default=[96, 48, 72])
     ap.add_argument("--box", type=int, nargs=3, default=[80, 64, 65])
     ap.add_argument("--max-steps", type=int, default=800)
+    ap.add_argument("--limiter-freeze", type=float, default=0.0,
+                    help="fix the limiter once the density residual drop reaches this level (0: never)")
     ap.add_argument("--krylov-max", type=int, default=300)
     ap.add_argument("--reorth", type=float, default=0.0, help="DGKS threshold (0: classical Gram-Schmidt once)")
     ap.add_argument("--no-repeat", action="store_true", default=True,
@@ -651,8 +715,14 @@ def main():
                     help="AMG hierarchy rebuilt at every n-th solve of the mixed AMG run")
     ap.add_argument("--restart", type=int, default=30, help="GMRES restart length of every implicit form")
     ap.add_argument("--drop", type=float, default=1e-8, help="residual drop of the time-to-drop runs")
-    ap.add_argument("--forms", default="ilu0,learned,amg",
-                    help="mixed forms of the time-to-drop runs besides the plain one (ilu0, learned, amg)")
+    ap.add_argument("--forms", default="ilu0,learned,learned_untrained,amg",
+                    help="mixed forms of the time-to-drop runs besides the plain one (ilu0, learned, "
+                         "learned_untrained, learned_handset, amg)")
+    ap.add_argument("--lp-train-steps", type=int, default=2, help="Adam steps per system of the learned training run")
+    ap.add_argument("--lp-train-pseudo", type=int, default=120,
+                    help="pseudo-time steps (systems) of the learned training run")
+    ap.add_argument("--lp-lr", type=float, default=3e-3, help="Adam learning rate of the learned training")
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false", help="keep the default launch shapes")
     ap.add_argument("--rtol", type=float, default=5e-2,
                     help="GMRES relative tolerance of every form (the outer loop needs no tighter)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,

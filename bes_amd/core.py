@@ -134,6 +134,8 @@ OPTIONAL_SIGS = {
     "bes_lp_train": [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_int, C.c_ulonglong, C.c_void_p],
     "bes_lp_fallbacks": [C.c_void_p],
     "bes_lp_set_online": [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_int],
+    "bes_set_limiter_freeze": [C.c_void_p, C.c_double],
+    "bes_limiter_frozen": [C.c_void_p, C.c_void_p],
     "bes_lp_set_rebuild_interval": [C.c_void_p, C.c_int],
     "bes_amg_set_params": [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int],
     "bes_amg_info": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int],
@@ -258,6 +260,16 @@ class Context:
         """
         _check(lib().bes_set_batch(self.h, int(nbatch)))
         self.batch = int(nbatch)
+
+    def set_limiter_freeze(self, drop):
+        """Fix each condition's limiter once its density residual drop reaches
+        `drop` in the implicit loop (0: never)."""
+        _check(lib().bes_set_limiter_freeze(self.h, float(drop)))
+
+    def limiter_frozen(self):
+        f = np.zeros(self.batch, np.int32)
+        _check(lib().bes_limiter_frozen(self.h, _p(f)))
+        return f.astype(bool)
 
     def set_freestream(self, primitive):
         """Freestream primitive [nv+1] for every condition, or [batch][nv+1]."""

@@ -77,6 +77,13 @@ struct bes_ctx {
   double *dtg = nullptr;       // [B] global dt
   bool dt_local = true;        // mode of the last time-step computation
   double *R = nullptr;         // [nv][N][B] residual (optional output)
+  // limiter freeze (bes_set_limiter_freeze): every gradient pass stores the
+  // limiter of the conditions not yet frozen in phi_frz; a frozen condition's
+  // passes read it back instead of recomputing it
+  double lim_freeze_drop = 0.0;
+  double *phi_frz = nullptr;         // [nv][N][B]
+  std::vector<char> lim_frozen;      // [B]
+  int lim_epoch = 0;                 // changes with the freeze state (graph captures hold it)
   // reduction scratch: [0, 64) global-dt minimum per condition, [64, 64 + nv*B)
   // Linf bit patterns [v][B]
   static constexpr int kRedWords = 64 + 5 * kMaxBatch;
@@ -87,7 +94,7 @@ struct bes_ctx {
   // pseudo-time step captured as a HIP graph (bes_explicit_graph)
   bool use_graph = false;
   hipGraphExec_t graph = nullptr;
-  int graph_key[4] = {0, 0, 0, 0};     // nstage, local, linf on, B of the capture
+  int graph_key[5] = {0, 0, 0, 0, 0};  // nstage, local, linf on, B, limiter-freeze epoch of the capture
   double graph_cfl = 0.0;
   std::vector<double> graph_alpha;
   int *dit = nullptr;                  // device iteration counter of graph replays

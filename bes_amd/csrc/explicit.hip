@@ -139,10 +139,13 @@ __device__ __forceinline__ double face_limiter(double mn, double mx, double d2, 
 // the first.  Every variable's arithmetic is the reference's, in its order.
 // Writes the cell pack group of variable v, [U_v | phi_v * grad U_v], at
 // ((i*NV + v)*GW + q)*B + b, GW = 1 + nd.
+// With frz non-null (limiter freeze on): a condition b with fmask.c[b] != 0
+// takes its limiter from frz, every other condition stores its limiter there.
 template <int ND>
 __global__ __launch_bounds__(64 * (ND + 2)) void k_gradlim(const double *__restrict__ U, DevMesh dm,
                                                           double *__restrict__ PK, double *__restrict__ Gout,
-                                                          double *__restrict__ Pout, int lgB) {
+                                                          double *__restrict__ Pout, double *__restrict__ frz,
+                                                          CoefB fmask, int lgB) {
   constexpr int NV = ND + 2, K = ND + 1, GW = ND + 1;
   const int N = dm.N, B = 1 << lgB;
   const int v = threadIdx.x >> 6;
@@ -187,18 +190,24 @@ __global__ __launch_bounds__(64 * (ND + 2)) void k_gradlim(const double *__restr
   const double V = dm.vol[i];
 #pragma unroll
   for (int d = 0; d < ND; d++) g[d] /= V;
-  const double eps2 = dm.eps2[i];
-  const double dmin = mn - u, dmax = mx - u;
   double best = 1.0;
+  const size_t pq = ((size_t)v * N + i) * B + b;
+  if (frz && fmask.c[b] != 0.0) {
+    best = frz[pq];
+  } else {
+    const double eps2 = dm.eps2[i];
+    const double dmin = mn - u, dmax = mx - u;
 #pragma unroll
-  for (int m = 0; m < K; m++) {
-    if (nb[m] < 0) continue;
-    double d2 = 0;
+    for (int m = 0; m < K; m++) {
+      if (nb[m] < 0) continue;
+      double d2 = 0;
 #pragma unroll
-    for (int d = 0; d < ND; d++) d2 += g[d] * dm.rl[((size_t)m * ND + d) * N + i];
-    d2 /= 2.0;
-    const double f = face_limiter(dmin, dmax, d2, eps2);
-    best = (f < best) ? f : best;
+      for (int d = 0; d < ND; d++) d2 += g[d] * dm.rl[((size_t)m * ND + d) * N + i];
+      d2 /= 2.0;
+      const double f = face_limiter(dmin, dmax, d2, eps2);
+      best = (f < best) ? f : best;
+    }
+    if (frz) frz[pq] = best;
   }
   double pk[GW];
   pk[0] = u;
@@ -207,7 +216,7 @@ __global__ __launch_bounds__(64 * (ND + 2)) void k_gradlim(const double *__restr
     if (Gout) Gout[(((size_t)v * ND + d) * N + i) * B + b] = g[d];
     pk[1 + d] = best * g[d];   // phi*grad: the product the reconstruction multiplies
   }
-  if (Pout) Pout[((size_t)v * N + i) * B + b] = best;
+  if (Pout) Pout[pq] = best;
   if (B == 1 && GW % 2 == 0) {   // the group is one aligned 32-byte run: 16-byte stores
     double2 *dst = reinterpret_cast<double2 *>(PK + ((size_t)i * NV + v) * GW);
 #pragma unroll
@@ -665,10 +674,16 @@ double gradlim_bytes(const bes_ctx *c) {
 int launch_gradlim(bes_ctx *c, const double *U, double *Gout, double *Pout) {
   const int nb = grid_for(c->N * c->B, 64);
   ProfScope ps(c, kProfGradlim, gradlim_bytes(c));
+  CoefB fm;
+  for (int b = 0; b < bes_ctx::kMaxBatch; b++)
+    fm.c[b] = (b < (int)c->lim_frozen.size() && c->lim_frozen[b]) ? 1.0 : 0.0;
+  double *frz = c->lim_freeze_drop > 0 ? c->phi_frz : nullptr;
   if (c->nd == 2)
-    hipLaunchKernelGGL(k_gradlim<2>, dim3(nb), dim3(64 * 4), 0, c->stream, U, c->dm, c->LG, Gout, Pout, c->lgB);
+    hipLaunchKernelGGL(k_gradlim<2>, dim3(nb), dim3(64 * 4), 0, c->stream, U, c->dm, c->LG, Gout, Pout, frz, fm,
+                       c->lgB);
   else
-    hipLaunchKernelGGL(k_gradlim<3>, dim3(nb), dim3(64 * 5), 0, c->stream, U, c->dm, c->LG, Gout, Pout, c->lgB);
+    hipLaunchKernelGGL(k_gradlim<3>, dim3(nb), dim3(64 * 5), 0, c->stream, U, c->dm, c->LG, Gout, Pout, frz, fm,
+                       c->lgB);
   BES_HIP(hipGetLastError());
   return BES_OK;
 }
@@ -756,11 +771,14 @@ static void drop_graph(bes_ctx *c) {
 static int resize_batch(bes_ctx *c, int B) {
   if (c->stream) BES_HIP(hipStreamSynchronize(c->stream));
   drop_graph(c);
-  double **arrs[] = {&c->U0, &c->Ua, &c->Ub, &c->LG, &c->FF, &c->dt, &c->G, &c->phi, &c->R, &c->linf};
+  double **arrs[] = {&c->U0, &c->Ua, &c->Ub, &c->LG, &c->FF, &c->dt, &c->G, &c->phi, &c->R, &c->linf, &c->phi_frz};
   for (double **p : arrs) {
     if (*p) hipFree(*p);
     *p = nullptr;
   }
+  c->lim_freeze_drop = 0.0;   // the freeze is per batch: set it again after a batch change
+  c->lim_frozen.clear();
+  c->lim_epoch++;
   c->linf_cap = 0;
   c->B = B;
   c->lgB = 0;
@@ -839,7 +857,7 @@ int bes_ctx_destroy(bes_ctx *c) {
   void *ptrs[] = {c->dm.vol, c->dm.eps2, c->dm.sproj, c->dm.nbr, c->dm.Sg, c->dm.w, c->dm.rl, c->dm.nf,
                   c->dm.af, c->dm.cfd, c->dm.info, c->dm.cfq, c->dm.saq, c->dm.fo, c->dm.fnb, c->dm.fnr,
                   c->dm.frs, c->dm.frn,
-                  c->FF, c->U0, c->Ua, c->Ub, c->LG, c->G, c->phi,
+                  c->FF, c->U0, c->Ua, c->Ub, c->LG, c->G, c->phi, c->phi_frz,
                   c->dt, c->dtg, c->R, c->red, c->err, c->linf, c->dfinf, c->dit};
   drop_graph(c);
   for (void *p : ptrs)
@@ -900,6 +918,26 @@ int bes_set_batch(bes_ctx *c, int nbatch) {
   if (nbatch == c->B) return BES_OK;
   bes_implicit_free(c);   // the implicit workspace is sized for one batch size
   return resize_batch(c, nbatch);
+}
+
+int bes_set_limiter_freeze(bes_ctx *c, double drop) {
+  if (!c) { set_error("null context"); return BES_ERR_ARG; }
+  if (!(drop >= 0.0 && drop < 1.0)) { set_error("limiter freeze drop must be in [0, 1)"); return BES_ERR_ARG; }
+  BES_HIP(hipStreamSynchronize(c->stream));
+  if (drop > 0.0 && !c->phi_frz) {
+    int s = dalloc(&c->phi_frz, (size_t)c->nv * c->N * c->B);
+    if (s) return s;
+  }
+  c->lim_freeze_drop = drop;
+  c->lim_frozen.assign(c->B, 0);
+  c->lim_epoch++;
+  return BES_OK;
+}
+
+int bes_limiter_frozen(bes_ctx *c, int *frozen) {
+  if (!c || !frozen) { set_error("invalid argument"); return BES_ERR_ARG; }
+  for (int b = 0; b < c->B; b++) frozen[b] = b < (int)c->lim_frozen.size() && c->lim_frozen[b];
+  return BES_OK;
 }
 
 int bes_get_batch(const bes_ctx *c) { return c ? c->B : 0; }
@@ -1018,8 +1056,8 @@ static int explicit_iteration(bes_ctx *c, int nstage, const double *alpha, doubl
 
 // capture one pseudo-time step as a graph (stage count > 1: the slots are fixed)
 static int capture_iteration(bes_ctx *c, int nstage, const double *alpha, double cfl, int local, bool linf) {
-  const int key[4] = {nstage, local, linf ? 1 : 0, c->B};
-  if (c->graph && std::equal(key, key + 4, c->graph_key) && c->graph_cfl == cfl &&
+  const int key[5] = {nstage, local, linf ? 1 : 0, c->B, c->lim_epoch};
+  if (c->graph && std::equal(key, key + 5, c->graph_key) && c->graph_cfl == cfl &&
       std::equal(alpha, alpha + nstage, c->graph_alpha.begin(), c->graph_alpha.end()))
     return BES_OK;
   drop_graph(c);
@@ -1035,7 +1073,7 @@ static int capture_iteration(bes_ctx *c, int nstage, const double *alpha, double
   e = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
   hipGraphDestroy(g);
   if (e != hipSuccess) { c->graph = nullptr; return hip_fail(e, "hipGraphInstantiate"); }
-  std::copy(key, key + 4, c->graph_key);
+  std::copy(key, key + 5, c->graph_key);
   c->graph_cfl = cfl;
   c->graph_alpha.assign(alpha, alpha + nstage);
   return BES_OK;

@@ -2125,6 +2125,14 @@ static int run_steps(bes_ctx *c, Impl *im, int nsteps, double *hist, int *steps_
       if (go[b]) {
         R.steps[b]++;
         if (R.r0[b] < 0) R.r0[b] = lrb[0];
+        // limiter freeze: from the step whose density residual first reaches
+        // the freeze drop on, the condition keeps the limiter of that step's
+        // gradient pass (stored by it in phi_frz)
+        if (c->lim_freeze_drop > 0 && b < (int)c->lim_frozen.size() && !c->lim_frozen[b] && R.r0[b] > 0 &&
+            lrb[0] <= c->lim_freeze_drop * R.r0[b]) {
+          c->lim_frozen[b] = 1;
+          c->lim_epoch++;
+        }
         // adaptive Courant number (density monitor of the implicit residual)
         if (R.cfl_max > R.cfl_min) {
           double &lo = R.lb_old[(size_t)b * nv];

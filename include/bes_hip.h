@@ -96,6 +96,19 @@ int bes_set_freestream(bes_ctx *ctx, const double *primitive);
  *      are the same bits as when it runs alone. ---- */
 int bes_set_batch(bes_ctx *ctx, int nbatch);
 int bes_get_batch(const bes_ctx *ctx);
+/* limiter freeze (no reference counterpart; the reference recomputes the
+ * Venkatakrishnan limiter at every pass, LimiterVenkatakrishnan.cpp:21-99).
+ * drop in (0, 1): the implicit loop (bes_implicit_run/_continue) fixes a
+ * condition's limiter values at those of the step whose density residual
+ * first reaches drop times its first value; later gradient passes of that
+ * condition (implicit and explicit) recompute the gradient but reuse those
+ * limiter values.  A frozen limiter makes the residual a smooth function of
+ * the state, which ends the limit cycles of a few limited cells that can hold
+ * the residual of a fine mesh at a fixed level.  drop 0 (the default) never
+ * freezes; any call clears the frozen state, and so does bes_set_batch.
+ * bes_limiter_frozen: per condition 1 if frozen ([nbatch]). */
+int bes_set_limiter_freeze(bes_ctx *ctx, double drop);
+int bes_limiter_frozen(bes_ctx *ctx, int *frozen);
 /* freestream primitive per condition, [nbatch][nv+1] */
 int bes_set_freestream_batch(bes_ctx *ctx, const double *primitive);
 
