@@ -136,6 +136,7 @@ OPTIONAL_SIGS = {
     "bes_lp_set_online": [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_int],
     "bes_set_limiter_freeze": [C.c_void_p, C.c_double],
     "bes_limiter_frozen": [C.c_void_p, C.c_void_p],
+    "bes_freeze_limiter_values": [C.c_void_p, C.c_void_p],
     "bes_lp_set_rebuild_interval": [C.c_void_p, C.c_int],
     "bes_amg_set_params": [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int],
     "bes_amg_info": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int],
@@ -265,6 +266,12 @@ class Context:
         """Fix each condition's limiter once its density residual drop reaches
         `drop` in the implicit loop (0: never)."""
         _check(lib().bes_set_limiter_freeze(self.h, float(drop)))
+
+    def freeze_limiter_values(self, phi):
+        """Fix every condition's limiter at phi (the layout gradient_limiter returns)."""
+        phi = np.ascontiguousarray(phi, dtype=np.float64)
+        assert phi.shape == self._shape(self.ncell, self.nv)
+        _check(lib().bes_freeze_limiter_values(self.h, _p(phi)))
 
     def limiter_frozen(self):
         f = np.zeros(self.batch, np.int32)

@@ -80,7 +80,8 @@ struct bes_ctx {
   // limiter freeze (bes_set_limiter_freeze): every gradient pass stores the
   // limiter of the conditions not yet frozen in phi_frz; a frozen condition's
   // passes read it back instead of recomputing it
-  double lim_freeze_drop = 0.0;
+  bool lim_freeze_on = false;        // phi_frz in use
+  double lim_freeze_drop = 0.0;      // drop that freezes a condition in the implicit loop (0: none)
   double *phi_frz = nullptr;         // [nv][N][B]
   std::vector<char> lim_frozen;      // [B]
   int lim_epoch = 0;                 // changes with the freeze state (graph captures hold it)

@@ -131,99 +131,127 @@ __device__ __forceinline__ double face_limiter(double mn, double mx, double d2, 
 }
 
 // Green-Gauss gradient (GradientGreenGauss.cpp:21-135) fused with the
-// Venkatakrishnan limiter (LimiterVenkatakrishnan.cpp:21-99).  One wave per
-// conserved variable: a block of NV waves covers 64 (cell, condition) pairs
-// and wave v computes variable v for all of them, so a thread holds one
-// variable's gradient and bounds (few registers, many waves in flight); the
-// waves of a block read the same geometry records, from the CU's cache after
-// the first.  Every variable's arithmetic is the reference's, in its order.
-// Writes the cell pack group of variable v, [U_v | phi_v * grad U_v], at
-// ((i*NV + v)*GW + q)*B + b, GW = 1 + nd.
+// Venkatakrishnan limiter (LimiterVenkatakrishnan.cpp:21-99).  One thread per
+// (cell, condition) computes every variable; all of its loads (geometry, the
+// cell's and its neighbours' states) are issued before the arithmetic that
+// needs them.  Every variable's arithmetic is the reference's, in its order.
+// Writes the cell pack, per variable the group [U_v | phi_v * grad U_v] at
+// ((i*NV + v)*GW + q)*B + b, GW = 1 + nd.  (One wave per variable instead
+// keeps fewer registers per thread but repeats the geometry loads and address
+// arithmetic per variable: 427 against 355 us on the 2M-tet box.)
 // With frz non-null (limiter freeze on): a condition b with fmask.c[b] != 0
 // takes its limiter from frz, every other condition stores its limiter there.
 template <int ND>
-__global__ __launch_bounds__(64 * (ND + 2)) void k_gradlim(const double *__restrict__ U, DevMesh dm,
-                                                          double *__restrict__ PK, double *__restrict__ Gout,
-                                                          double *__restrict__ Pout, double *__restrict__ frz,
-                                                          CoefB fmask, int lgB) {
+__global__ __launch_bounds__(256) void k_gradlim(const double *__restrict__ U, DevMesh dm,
+                                                    double *__restrict__ PK, double *__restrict__ Gout,
+                                                    double *__restrict__ Pout, double *__restrict__ frz,
+                                                    CoefB fmask, int lgB) {
   constexpr int NV = ND + 2, K = ND + 1, GW = ND + 1;
   const int N = dm.N, B = 1 << lgB;
-  const int v = threadIdx.x >> 6;
-  const int t = xcd_block() * 64 + (threadIdx.x & 63);
+  const int t = cell_index();
   const int i = t >> lgB, b = t & (B - 1);
   if (i >= N) return;
-  const double u = U[((size_t)v * N + i) * B + b];
-  double mn = u, mx = u, g[ND];
-#pragma unroll
-  for (int d = 0; d < ND; d++) g[d] = 0.0;
   int nb[K];
+  double S[K][ND], w[K], u[NV], uj[K][NV];
 #pragma unroll
   for (int m = 0; m < K; m++) {
-    const int j = dm.nbr[(size_t)m * N + i];
-    nb[m] = j;
-    double S[ND];
+    nb[m] = dm.nbr[(size_t)m * N + i];
+    w[m] = dm.w[(size_t)m * N + i];
 #pragma unroll
-    for (int d = 0; d < ND; d++) S[d] = dm.Sg[((size_t)m * ND + d) * N + i];
-    if (j >= 0) {
-      const double w = dm.w[(size_t)m * N + i];
-      const double uj = U[((size_t)v * N + j) * B + b];
-      const double fv = w * u + (1 - w) * uj;
+    for (int d = 0; d < ND; d++) S[m][d] = dm.Sg[((size_t)m * ND + d) * N + i];
+  }
 #pragma unroll
-      for (int d = 0; d < ND; d++) g[d] += S[d] * fv;
-      mn = (uj < mn) ? uj : mn;
-      mx = (uj > mx) ? uj : mx;
+  for (int v = 0; v < NV; v++) u[v] = U[((size_t)v * N + i) * B + b];
+#pragma unroll
+  for (int m = 0; m < K; m++) {
+    const int j = nb[m] >= 0 ? nb[m] : i;
+#pragma unroll
+    for (int v = 0; v < NV; v++) uj[m][v] = U[((size_t)v * N + j) * B + b];
+  }
+  double g[NV][ND], mn[NV], mx[NV];
+#pragma unroll
+  for (int v = 0; v < NV; v++) {
+    mn[v] = u[v];
+    mx[v] = u[v];
+#pragma unroll
+    for (int d = 0; d < ND; d++) g[v][d] = 0.0;
+  }
+#pragma unroll
+  for (int m = 0; m < K; m++) {
+    if (nb[m] >= 0) {
+#pragma unroll
+      for (int v = 0; v < NV; v++) {
+        const double fv = w[m] * u[v] + (1 - w[m]) * uj[m][v];
+#pragma unroll
+        for (int d = 0; d < ND; d++) g[v][d] += S[m][d] * fv;
+        mn[v] = (uj[m][v] < mn[v]) ? uj[m][v] : mn[v];
+        mx[v] = (uj[m][v] > mx[v]) ? uj[m][v] : mx[v];
+      }
     } else {
 #pragma unroll
-      for (int d = 0; d < ND; d++) g[d] += S[d] * (u);
-      if (j == -1 && v >= 1 && v <= ND) {   // slip wall: tangential momentum on the face
+      for (int v = 0; v < NV; v++)
+#pragma unroll
+        for (int d = 0; d < ND; d++) g[v][d] += S[m][d] * (u[v]);
+      if (nb[m] == -1) {   // slip wall: tangential momentum on the face
         double nn[ND], vn = 0;
 #pragma unroll
         for (int d = 0; d < ND; d++) nn[d] = dm.nf[((size_t)m * ND + d) * N + i];
 #pragma unroll
-        for (int d = 0; d < ND; d++) vn += U[((size_t)(1 + d) * N + i) * B + b] * nn[d];
-        const double tg = u - vn * nn[v - 1];
+        for (int d = 0; d < ND; d++) vn += u[1 + d] * nn[d];
 #pragma unroll
-        for (int d = 0; d < ND; d++) g[d] += (tg * S[d] - u * S[d]);
+        for (int a = 0; a < ND; a++) {
+          const double tg = u[1 + a] - vn * nn[a];
+#pragma unroll
+          for (int d = 0; d < ND; d++) g[1 + a][d] += (tg * S[m][d] - u[1 + a] * S[m][d]);
+        }
       }
     }
   }
   const double V = dm.vol[i];
+  const double eps2 = dm.eps2[i];
+  double rl[K][ND];
 #pragma unroll
-  for (int d = 0; d < ND; d++) g[d] /= V;
-  double best = 1.0;
-  const size_t pq = ((size_t)v * N + i) * B + b;
-  if (frz && fmask.c[b] != 0.0) {
-    best = frz[pq];
-  } else {
-    const double eps2 = dm.eps2[i];
-    const double dmin = mn - u, dmax = mx - u;
+  for (int m = 0; m < K; m++)
 #pragma unroll
-    for (int m = 0; m < K; m++) {
-      if (nb[m] < 0) continue;
-      double d2 = 0;
+    for (int d = 0; d < ND; d++) rl[m][d] = dm.rl[((size_t)m * ND + d) * N + i];
 #pragma unroll
-      for (int d = 0; d < ND; d++) d2 += g[d] * dm.rl[((size_t)m * ND + d) * N + i];
-      d2 /= 2.0;
-      const double f = face_limiter(dmin, dmax, d2, eps2);
-      best = (f < best) ? f : best;
+  for (int v = 0; v < NV; v++) {
+#pragma unroll
+    for (int d = 0; d < ND; d++) g[v][d] /= V;
+    double best = 1.0;
+    const size_t pq = ((size_t)v * N + i) * B + b;
+    if (frz && fmask.c[b] != 0.0) {
+      best = frz[pq];
+    } else {
+      const double dmin = mn[v] - u[v], dmax = mx[v] - u[v];
+#pragma unroll
+      for (int m = 0; m < K; m++) {
+        if (nb[m] < 0) continue;
+        double d2 = 0;
+#pragma unroll
+        for (int d = 0; d < ND; d++) d2 += g[v][d] * rl[m][d];
+        d2 /= 2.0;
+        const double f = face_limiter(dmin, dmax, d2, eps2);
+        best = (f < best) ? f : best;
+      }
+      if (frz) frz[pq] = best;
     }
-    if (frz) frz[pq] = best;
-  }
-  double pk[GW];
-  pk[0] = u;
+    double pk[GW];
+    pk[0] = u[v];
 #pragma unroll
-  for (int d = 0; d < ND; d++) {
-    if (Gout) Gout[(((size_t)v * ND + d) * N + i) * B + b] = g[d];
-    pk[1 + d] = best * g[d];   // phi*grad: the product the reconstruction multiplies
-  }
-  if (Pout) Pout[pq] = best;
-  if (B == 1 && GW % 2 == 0) {   // the group is one aligned 32-byte run: 16-byte stores
-    double2 *dst = reinterpret_cast<double2 *>(PK + ((size_t)i * NV + v) * GW);
+    for (int d = 0; d < ND; d++) {
+      if (Gout) Gout[(((size_t)v * ND + d) * N + i) * B + b] = g[v][d];
+      pk[1 + d] = best * g[v][d];
+    }
+    if (Pout) Pout[pq] = best;
+    if (B == 1 && GW % 2 == 0) {
+      double2 *dst = reinterpret_cast<double2 *>(PK + ((size_t)i * NV + v) * GW);
 #pragma unroll
-    for (int q = 0; q < GW / 2; q++) dst[q] = make_double2(pk[2 * q], pk[2 * q + 1]);
-  } else {
+      for (int q = 0; q < GW / 2; q++) dst[q] = make_double2(pk[2 * q], pk[2 * q + 1]);
+    } else {
 #pragma unroll
-    for (int q = 0; q < GW; q++) PK[(((size_t)i * NV + v) * GW + q) * B + b] = pk[q];
+      for (int q = 0; q < GW; q++) PK[(((size_t)i * NV + v) * GW + q) * B + b] = pk[q];
+    }
   }
 }
 
@@ -670,19 +698,19 @@ double gradlim_bytes(const bes_ctx *c) {
   return N * (8 * nv * B + k * (4 + 16 * nd + 8) + 16 + 8 * nv * (1 + nd) * B);
 }
 
-// one block per 64 (cell, condition) pairs, one wave per conserved variable
+// one thread per (cell, condition)
 int launch_gradlim(bes_ctx *c, const double *U, double *Gout, double *Pout) {
-  const int nb = grid_for(c->N * c->B, 64);
+  const int bs = tuned_bs(c, kProfGradlim), nb = grid_for(c->N * c->B, bs);
   ProfScope ps(c, kProfGradlim, gradlim_bytes(c));
   CoefB fm;
   for (int b = 0; b < bes_ctx::kMaxBatch; b++)
     fm.c[b] = (b < (int)c->lim_frozen.size() && c->lim_frozen[b]) ? 1.0 : 0.0;
-  double *frz = c->lim_freeze_drop > 0 ? c->phi_frz : nullptr;
+  double *frz = c->lim_freeze_on ? c->phi_frz : nullptr;
   if (c->nd == 2)
-    hipLaunchKernelGGL(k_gradlim<2>, dim3(nb), dim3(64 * 4), 0, c->stream, U, c->dm, c->LG, Gout, Pout, frz, fm,
+    hipLaunchKernelGGL(k_gradlim<2>, dim3(nb), dim3(bs), 0, c->stream, U, c->dm, c->LG, Gout, Pout, frz, fm,
                        c->lgB);
   else
-    hipLaunchKernelGGL(k_gradlim<3>, dim3(nb), dim3(64 * 5), 0, c->stream, U, c->dm, c->LG, Gout, Pout, frz, fm,
+    hipLaunchKernelGGL(k_gradlim<3>, dim3(nb), dim3(bs), 0, c->stream, U, c->dm, c->LG, Gout, Pout, frz, fm,
                        c->lgB);
   BES_HIP(hipGetLastError());
   return BES_OK;
@@ -777,6 +805,7 @@ static int resize_batch(bes_ctx *c, int B) {
     *p = nullptr;
   }
   c->lim_freeze_drop = 0.0;   // the freeze is per batch: set it again after a batch change
+  c->lim_freeze_on = false;
   c->lim_frozen.clear();
   c->lim_epoch++;
   c->linf_cap = 0;
@@ -929,7 +958,24 @@ int bes_set_limiter_freeze(bes_ctx *c, double drop) {
     if (s) return s;
   }
   c->lim_freeze_drop = drop;
+  c->lim_freeze_on = drop > 0.0;
   c->lim_frozen.assign(c->B, 0);
+  c->lim_epoch++;
+  return BES_OK;
+}
+
+int bes_freeze_limiter_values(bes_ctx *c, const double *phi) {
+  if (!c || !phi) { set_error("null argument"); return BES_ERR_ARG; }
+  BES_HIP(hipStreamSynchronize(c->stream));
+  if (!c->phi_frz) {
+    int s = dalloc(&c->phi_frz, (size_t)c->nv * c->N * c->B);
+    if (s) return s;
+  }
+  int s = state_to_device(c, phi, c->nv, c->phi_frz);
+  if (s) return s;
+  c->lim_freeze_on = true;
+  c->lim_freeze_drop = 0.0;
+  c->lim_frozen.assign(c->B, 1);
   c->lim_epoch++;
   return BES_OK;
 }

@@ -2243,7 +2243,7 @@ int bes_autotune(bes_ctx *c, int flags, int *chosen, double *best_us, int *color
   }
   if (!s && (flags & 1)) {   // explicit kernels on the current state, last: the time step it leaves is at CFL 1
     // (U0 is read, never written; the next assembly or explicit step recomputes dt)
-    s = launch_gradlim(c, c->U0, nullptr, nullptr);
+    s = tune(kProfGradlim, [&]() { return launch_gradlim(c, c->U0, nullptr, nullptr); });
     if (!s) s = tune(kProfTimestep, [&]() { return launch_timestep(c, c->U0, 1.0, 1); });
     if (!s) s = tune(kProfFaceFlux, [&]() { return launch_face_flux(c); });
     if (!s) s = tune(kProfResidual, [&]() { return launch_residual(c, nullptr, nullptr, 0.0, 1, nullptr, nullptr); });

@@ -109,6 +109,12 @@ int bes_get_batch(const bes_ctx *ctx);
  * bes_limiter_frozen: per condition 1 if frozen ([nbatch]). */
 int bes_set_limiter_freeze(bes_ctx *ctx, double drop);
 int bes_limiter_frozen(bes_ctx *ctx, int *frozen);
+/* freeze every condition's limiter at given values, phi [nbatch][ncell][nv] as
+ * bes_get_limiter returns them (no reference counterpart): later gradient
+ * passes recompute the gradient and use these values, so that several solver
+ * forms converge the same frozen discretisation.  bes_set_limiter_freeze
+ * (any drop) ends it. */
+int bes_freeze_limiter_values(bes_ctx *ctx, const double *phi);
 /* freestream primitive per condition, [nbatch][nv+1] */
 int bes_set_freestream_batch(bes_ctx *ctx, const double *primitive);
 
@@ -355,8 +361,8 @@ int bes_set_profiling(bes_ctx *ctx, int on);
 /* autotuner (no reference counterpart; the reference's counterpart is its
  * OpenMP schedule, SolverExplicit.cpp:285): times each tunable kernel kind at
  * block sizes 64, 128 and 256 on the context's current data with HIP events
- * and keeps the fastest.  flags: 1 the explicit kernels (time step, face flux,
- * residual) on the current state; 2 the implicit kernels (Jacobians, assembly,
+ * and keeps the fastest.  flags: 1 the explicit kernels (gradient and
+ * limiter, time step, face flux, residual) on the current state; 2 the implicit kernels (Jacobians, assembly,
  * factorisation, SpMV, preconditioner, update) on the assembled system, which
  * is assembled and factored again (same values); 4 the ILU(0) colouring order
  * (bes_implicit_set_coloring), chosen as the one whose solve of the assembled

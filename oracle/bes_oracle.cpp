@@ -837,6 +837,7 @@ struct or_implicit {
   std::vector<std::vector<long>> rows;      // rows of each colour, ascending
   std::vector<double> A, LU, Dinv, b, R, x, dt;
   std::vector<double> V, Zv, w, u;          // Krylov basis [(m+1)][n], work vectors
+  std::vector<double> phi_frozen;           // [ne][nv] limiter used instead of recomputing it (if set)
   long last_iters = 0;
 };
 
@@ -1095,7 +1096,8 @@ void imp_assemble(const or_mesh *m, or_implicit *s, const double *U, double cfl,
   std::vector<double> G(ne * nv * nd), P(ne * nv);
   or_timestep(m, U, cfl, local, gas, s->dt.data());   // serial, as TimeStepCalculatorLocal
   or_gradient(m, U, G.data());
-  or_limiter(m, U, G.data(), gas, P.data());
+  if (s->phi_frozen.empty()) or_limiter(m, U, G.data(), gas, P.data());
+  else P = s->phi_frozen;
   std::fill(s->A.begin(), s->A.end(), 0.0);
   for (long e = 0; e < ne; e++)
     for (int a = 0; a < nv; a++) s->A[s->rowptr[e] * nb + a * nv + a] = 1.0;
@@ -1197,6 +1199,11 @@ or_implicit *or_implicit_create(const or_mesh *m, int restart, int maxiter, doub
 void or_implicit_destroy(or_implicit *s) { delete s; }
 
 int or_implicit_ncolor(const or_implicit *s) { return (int)s->rows.size(); }
+
+void or_implicit_freeze_limiter(or_implicit *s, const double *phi) {
+  if (phi) s->phi_frozen.assign(phi, phi + s->ne * s->nv);
+  else s->phi_frozen.clear();
+}
 
 int or_implicit_steps(or_implicit *s, const or_mesh *m, double *U, int nsteps, double *cfl, double cfl_min,
                       double cfl_max, const double *ex, int local, int repeat_nv, double stop_drop, double r0,

@@ -96,6 +96,10 @@ int or_implicit_steps(or_implicit *s, const or_mesh *m, double *U, int nsteps, d
 /* the last step's matrix (BSR values as or_implicit_assemble), right-hand side
  * and linear solution; any may be NULL */
 void or_implicit_last(const or_implicit *s, double *bval, double *rhs, double *x);
+/* limiter values [ncell][nv] that every later assembly uses instead of
+ * recomputing the limiter (the product's limiter freeze; no reference
+ * counterpart), or NULL to recompute again */
+void or_implicit_freeze_limiter(or_implicit *s, const double *phi);
 
 #ifdef __cplusplus
 }

@@ -71,6 +71,7 @@ def lib():
         _lib.or_implicit_steps.argtypes = [P, P, P, C.c_int, P, C.c_double, C.c_double, P, C.c_int, C.c_int,
                                            C.c_double, C.c_double, P, P, P, P]
         _lib.or_implicit_last.argtypes = [P, P, P, P]
+        _lib.or_implicit_freeze_limiter.argtypes = [P, P]
     return _lib
 
 
@@ -248,6 +249,12 @@ class Implicit:
         if self.r0 <= 0 and n > 0:
             self.r0 = float(hist[0, nv])
         return hist[:n]
+
+    def freeze_limiter(self, phi):
+        """Use the limiter values phi [ncell][nv] in every later assembly
+        (None: recompute the limiter again)."""
+        self._phi = None if phi is None else np.ascontiguousarray(phi, dtype=np.float64)
+        lib().or_implicit_freeze_limiter(self.h, _p(self._phi))
 
     def last_system(self):
         nv = self.mesh.nv

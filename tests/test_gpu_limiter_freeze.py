@@ -109,3 +109,33 @@ def test_freeze_argument_checks():
         c.set_limiter_freeze(0.5)
         c.set_batch(2)   # a batch change clears the freeze
         assert not c.limiter_frozen().any()
+
+
+def test_freeze_at_given_values():
+    """bes_freeze_limiter_values: every later gradient pass uses the given
+    limiter (so several solver forms converge one frozen discretisation)."""
+    md = mg.swept_wing(12, 6, 8)
+    P, U0 = _start(md)
+    rng = np.random.default_rng(11)
+    with core.Context(md) as c:
+        c.set_freestream(P)
+        c.set_state(U0 * (1 + 1e-2 * rng.uniform(-1, 1, U0.shape)))
+        G0, P0 = c.gradient_limiter()
+        c.set_state(U0 * (1 + 1e-2 * rng.uniform(-1, 1, U0.shape)))
+        G1, P1 = c.gradient_limiter()
+        assert not np.array_equal(P0, P1)
+        c.freeze_limiter_values(P0)
+        assert c.limiter_frozen().all()
+        G2, P2 = c.gradient_limiter()
+        assert np.array_equal(P2, P0) and np.array_equal(G2, G1)
+        c.implicit_setup(64, "ilu0", rtol=5e-2, maxiter=200)
+        c.implicit_run(3, *SCHEDULE, repeat_nv=False)
+        assert np.array_equal(c.gradient_limiter()[1], P0)
+        c.set_limiter_freeze(0.0)
+        assert not c.limiter_frozen().any()
+        U = c.get_state()
+        P3 = c.gradient_limiter()[1]
+    with core.Context(md) as c:
+        c.set_freestream(P)
+        c.set_state(U)
+        assert np.array_equal(c.gradient_limiter()[1], P3)

@@ -118,3 +118,34 @@ def test_reference_iterations_are_the_stage_composition(golden):
     for _ in range(int(g["niter"])):
         U = _stage_composition(om, U, float(g["cfl"]), finf, orc.RK4_ALPHA)
     assert np.array_equal(U, g["explicit_state"]), name
+
+
+def test_implicit_loop_with_frozen_limiter():
+    """The oracle's limiter freeze (the product's bes_freeze_limiter_values)
+    replaces only the limiter: one step with the start state's own limiter
+    frozen is bit for bit the unfrozen step; a second step is not (its state's
+    limiter differs), and a frozen run keeps converging."""
+    from bes_amd import core, meshgen as mg
+    md = mg.swept_wing(8, 4, 6)
+    finf = core.freestream_primitive(core.wing_freestream(0.84, 3.06), 3)
+    U0 = np.tile(core.conserved_from_primitive(finf, 3), (md.ncell, 1))
+    U0 = U0 * (1 + 0.01 * np.random.default_rng(2).uniform(-1, 1, U0.shape))
+    om = orc.Mesh(md)
+    phi0 = om.limiter(U0, om.gradient(U0))
+
+    def run(n, phi):
+        s = orc.Implicit(om, rtol=5e-2, maxiter=100, nthreads=1)
+        s.freeze_limiter(phi)
+        s.start(1.0, 1000.0, (0.5, 1.5))
+        U = np.ascontiguousarray(U0.copy())
+        h = s.steps(U, n, finf)
+        return U, h
+
+    Ua, ha = run(1, None)
+    Ub, hb = run(1, phi0)
+    assert np.array_equal(Ua, Ub) and np.array_equal(ha, hb)
+    Ua, ha = run(2, None)
+    Ub, hb = run(2, phi0)
+    assert np.array_equal(ha[0], hb[0]) and not np.array_equal(ha[1], hb[1])
+    U, h = run(30, phi0)
+    assert h[-1, 5] < 1e-2 * h[0, 5]
