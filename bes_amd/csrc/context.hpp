@@ -89,6 +89,7 @@ struct bes_ctx {
   // Linf bit patterns [v][B]
   static constexpr int kRedWords = 64 + 5 * kMaxBatch;
   unsigned long long *red = nullptr;
+  unsigned long long *lbits = nullptr;  // Linf monitor words [kLinfSlots][nv][B] (see kernels_common.hpp)
   unsigned long long *err = nullptr;   // first failing device face, ~0 if none
   double *linf = nullptr;              // [cap][B][nv] per-iteration Linf
   int linf_cap = 0;

@@ -396,7 +396,7 @@ __global__ __launch_bounds__(256) void k_cell_residual(
     for (int v = 0; v < NV; v++) {
       double a = block_max_cond(::fabs(R[v]), B);
       if ((int)threadIdx.x < B)
-        atomicMax(&linf_bits[v * B + threadIdx.x], (unsigned long long)__double_as_longlong(a));
+        atomicMax(&linf_slot(linf_bits, NV * B)[v * B + threadIdx.x], (unsigned long long)__double_as_longlong(a));
     }
   }
 }
@@ -415,9 +415,18 @@ __global__ void k_linf_decode(const unsigned long long *bits, double *out, int n
   __syncthreads();
   if (it < cap && q < nv * B) {
     const int v = q / B, b = q - v * B;
-    out[((size_t)it * B + b) * nv + v] = __longlong_as_double((long long)bits[q]);
+    out[((size_t)it * B + b) * nv + v] = __longlong_as_double((long long)linf_slots_max(bits, nv * B, q));
   }
   if (dit && q == 0) *dit = it + 1;
+}
+
+// bits[q] = max over the Linf copies (copy 0 in place)
+__global__ void k_linf_reduce(unsigned long long *bits, int nq) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < nq) {
+    const unsigned long long m = linf_slots_max(bits, nq, q);
+    bits[q] = m;
+  }
 }
 
 // per-face flux / Jacobian batches (host-facing known-answer interface)
@@ -633,6 +642,7 @@ static int create_from_input(const MeshInput &in, int geom_mode, const bes_gas *
       (s = dalloc(&c->Ub, nv * N)) || (s = dalloc(&c->LG, nv * (1 + nd) * N)) ||
       (s = dalloc(&c->FF, (size_t)c->fs * c->dm.F)) || (s = dalloc(&c->dt, N)) ||
       (s = dalloc(&c->dtg, bes_ctx::kMaxBatch)) || (s = dalloc(&c->red, bes_ctx::kRedWords)) ||
+      (s = dalloc(&c->lbits, (size_t)kLinfSlots * 5 * bes_ctx::kMaxBatch)) ||
       (s = dalloc(&c->err, 1)) || (s = dalloc(&c->dfinf, bes_ctx::kMaxBatch * 6)) ||
       (s = dalloc(&c->dit, 1))) {
     bes_ctx_destroy(c);
@@ -887,7 +897,7 @@ int bes_ctx_destroy(bes_ctx *c) {
                   c->dm.af, c->dm.cfd, c->dm.info, c->dm.cfq, c->dm.saq, c->dm.fo, c->dm.fnb, c->dm.fnr,
                   c->dm.frs, c->dm.frn,
                   c->FF, c->U0, c->Ua, c->Ub, c->LG, c->G, c->phi, c->phi_frz,
-                  c->dt, c->dtg, c->R, c->red, c->err, c->linf, c->dfinf, c->dit};
+                  c->dt, c->dtg, c->R, c->red, c->lbits, c->err, c->linf, c->dfinf, c->dit};
   drop_graph(c);
   for (void *p : ptrs)
     if (p) hipFree(p);
@@ -1060,10 +1070,11 @@ int bes_compute_residual(bes_ctx *c, double *R_out, double *maxabs_out) {
   if (!c) { set_error("null context"); return BES_ERR_ARG; }
   if (!c->R) { int s = dalloc(&c->R, (size_t)c->nv * c->N * c->B); if (s) return s; }
   const int nv = c->nv, B = c->B;
-  unsigned long long *bits = c->red + 64;
+  unsigned long long *bits = c->lbits;
   int s;
-  if ((s = reset_err(c)) || (s = fill_bits(c, bits, nv * B, 0ull))) return s;
+  if ((s = reset_err(c)) || (s = fill_bits(c, bits, kLinfSlots * nv * B, 0ull))) return s;
   if ((s = launch_residual(c, nullptr, nullptr, 0.0, 1, c->R, bits))) return s;
+  hipLaunchKernelGGL(k_linf_reduce, dim3(grid_for(nv * B, 64)), dim3(64), 0, c->stream, bits, nv * B);
   if (maxabs_out) {
     std::vector<unsigned long long> b(nv * B);
     BES_HIP(hipMemcpyAsync(b.data(), bits, b.size() * 8, hipMemcpyDeviceToHost, c->stream));
@@ -1080,7 +1091,7 @@ int bes_compute_residual(bes_ctx *c, double *R_out, double *maxabs_out) {
 static int explicit_iteration(bes_ctx *c, int nstage, const double *alpha, double cfl, int local, bool linf,
                               int it_host) {
   const int nv = c->nv, B = c->B;
-  unsigned long long *bits = c->red + 64;
+  unsigned long long *bits = c->lbits;
   int s;
   if ((s = launch_timestep(c, c->U0, cfl, local))) return s;
   for (int st = 0; st < nstage; st++) {
@@ -1088,7 +1099,7 @@ static int explicit_iteration(bes_ctx *c, int nstage, const double *alpha, doubl
     const double *src = st == 0 ? c->U0 : ((st - 1) % 2 == 0 ? c->Ua : c->Ub);
     double *dst = (last && nstage > 1) ? c->U0 : (st % 2 == 0 ? c->Ua : c->Ub);
     if ((s = launch_gradlim(c, src, nullptr, nullptr))) return s;
-    if (last && linf && (s = fill_bits(c, bits, nv * B, 0ull))) return s;
+    if (last && linf && (s = fill_bits(c, bits, kLinfSlots * nv * B, 0ull))) return s;
     if ((s = launch_residual(c, c->U0, dst, alpha[st], local, nullptr, (last && linf) ? bits : nullptr))) return s;
     if (last && linf) {
       hipLaunchKernelGGL(k_linf_decode, dim3(1), dim3(nv * B), 0, c->stream, bits, c->linf, nv, B,

@@ -210,9 +210,11 @@ __global__ __launch_bounds__(256) void k_assemble(
 #pragma unroll
     for (int v = 0; v < NV; v++) {
       double a = block_max_cond(::fabs(bi[v]), B);
-      if ((int)threadIdx.x < B) atomicMax(&linf_b[v * B + threadIdx.x], (unsigned long long)__double_as_longlong(a));
+      if ((int)threadIdx.x < B)
+        atomicMax(&linf_slot(linf_b, 16 * B)[v * B + threadIdx.x], (unsigned long long)__double_as_longlong(a));
       double c = block_max_cond(::fabs(ri[v]), B);
-      if ((int)threadIdx.x < B) atomicMax(&linf_r[v * B + threadIdx.x], (unsigned long long)__double_as_longlong(c));
+      if ((int)threadIdx.x < B)
+        atomicMax(&linf_slot(linf_r, 16 * B)[v * B + threadIdx.x], (unsigned long long)__double_as_longlong(c));
     }
   }
 }
@@ -928,6 +930,15 @@ __global__ __launch_bounds__(256) void k_update(double *__restrict__ U, const do
   for (int v = 0; v < NV; v++) U[((size_t)v * N + i) * B + b] += x[((size_t)i * NV + v) * B + b];
 }
 
+// bits[q] = max over the Linf copies (copy 0 in place)
+__global__ void k_linf_reduce_i(unsigned long long *bits, int nq) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < nq) {
+    const unsigned long long m = linf_slots_max(bits, nq, q);
+    bits[q] = m;
+  }
+}
+
 __global__ void k_fill_u64_i(unsigned long long *p, int n, unsigned long long v) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
@@ -1144,7 +1155,7 @@ static int impl_alloc(bes_ctx *c, Impl *im) {
   BES_HIP(hipMalloc((void **)&im->dots, 64 * Bs * sizeof(double)));
   BES_HIP(hipHostMalloc((void **)&im->hdots, 64 * Bs * sizeof(double), hipHostMallocDefault));
   BES_HIP(hipEventCreateWithFlags(&im->ev_dots, hipEventDisableTiming));
-  BES_HIP(hipMalloc((void **)&im->bits, 16 * Bs * sizeof(unsigned long long)));
+  BES_HIP(hipMalloc((void **)&im->bits, (size_t)kLinfSlots * 16 * Bs * sizeof(unsigned long long)));
   return BES_OK;
 }
 
@@ -1182,7 +1193,8 @@ static int launch_assemble(bes_ctx *c, Impl *im) {
                          im->fd, c->gc, c->dfinf, (TM *)im->JF, lgB);
     BES_HIP(hipGetLastError());
   }
-  hipLaunchKernelGGL(k_fill_u64_i, dim3((16 * B + 63) / 64), dim3(64), 0, c->stream, im->bits, 16 * B, 0ull);
+  hipLaunchKernelGGL(k_fill_u64_i, dim3((kLinfSlots * 16 * B + 63) / 64), dim3(64), 0, c->stream, im->bits,
+                     kLinfSlots * 16 * B, 0ull);
   int s = launch_face_flux(c);
   if (s) return s;
   // per cell: dt, volume, face records, the cell's face fluxes and both
@@ -1839,6 +1851,7 @@ static int assemble(bes_ctx *c, Impl *im, double cfl, const double *cfl_b, int l
   if (im->prec == 64) s = launch_assemble<double>(c, im);
   else s = launch_assemble<float>(c, im);
   if (s) return s;
+  hipLaunchKernelGGL(k_linf_reduce_i, dim3(grid_for(16 * B, 64)), dim3(64), 0, c->stream, im->bits, 16 * B);
   std::vector<unsigned long long> bb((size_t)16 * B);
   BES_HIP(hipMemcpyAsync(bb.data(), im->bits, bb.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                          c->stream));

@@ -71,6 +71,24 @@ __device__ __forceinline__ double block_min(double v) {
 // 64).  Lanes of a wave with the same condition are combined by xor-shuffles
 // with offsets >= B, then the waves through LDS; thread t < B holds the
 // result for condition t.  Every thread of the block must call them.
+// Linf monitors: a launch's blocks add their maxima with atomicMax into
+// kLinfSlots copies of the monitor words (block b into copy b mod kLinfSlots),
+// not one, so thousands of blocks do not queue on a few addresses; a reduce
+// over the copies follows (max is exact, so the order does not matter).
+constexpr int kLinfSlots = 64;
+__device__ __forceinline__ unsigned long long *linf_slot(unsigned long long *bits, int nq) {
+  return bits + (size_t)(blockIdx.x & (kLinfSlots - 1)) * nq;
+}
+// out[q] = max over the copies of bits[s*nq + q] (out may be bits: copy 0)
+__device__ __forceinline__ unsigned long long linf_slots_max(const unsigned long long *bits, int nq, int q) {
+  unsigned long long m = 0;
+  for (int s = 0; s < kLinfSlots; s++) {
+    const unsigned long long x = bits[(size_t)s * nq + q];
+    m = x > m ? x : m;
+  }
+  return m;
+}
+
 __device__ __forceinline__ double block_max_cond(double v, int B) {
   __shared__ double s[16][64];
   for (int o = 32; o >= B; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
