@@ -646,13 +646,58 @@ def implicit_sweep_workload(args):
            "launch": f"batches of {B} conditions sorted by Mach number, one launch per kernel for a whole batch; "
                      f"ILU(0) in the mesh-order colouring"}
     reached = sum(r["reached"] for r in runs)
+    amg = None
+    if args.isweep_amg:
+        amg = implicit_sweep_amg(args, md, mach, aoa, freeze=args.limiter_freeze if args.limiter_freeze > 0 else 1e-2)
     return {"metric": METRIC, "value": total_cu / wall_all, "unit": "cell-updates/s",
             "n_gpus": 1, "steps": int(sum(r["steps"] for r in runs)), "warmup": 0,
             "ms_per_step": wall_all * 1e3 / max(1, sum(r["steps"] for r in runs)) * B,
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
             "dtype": "f64 residual, f32 solve", "data": "synthetic", "config": cfg,
             "sweep": {"wall_s": wall_all, "conditions_reached": reached, "s_per_condition": wall_all / len(mach),
-                      "runs": runs}}
+                      "runs": runs},
+            "sweep_amg_one_at_a_time": amg}
+
+
+def implicit_sweep_amg(args, md, mach, aoa, freeze):
+    """The same conditions one at a time on one context with the fastest
+    single-condition form (FP32 aggregation AMG-GMRES, the wing's CFL cap):
+    the comparison for the batched ILU(0) sweep, whose batches iterate until
+    their hardest condition converges."""
+    from bes_amd import core, meshgen as mg
+    cmin, cmax, ex = WING_CFL
+    runs, wall_all = [], 0.0
+    with core.Context(md) as c:
+        geo = c.geometry()
+        centers = _centers(md, geo)
+        c.implicit_setup(32, "amg", restart=args.restart, maxiter=args.krylov_max, rtol=args.amg_rtol)
+        c.implicit_set_reorth(args.reorth)
+        for q in range(len(mach)):
+            # new parameters force a fresh hierarchy: no lagged one from the last condition
+            c.amg_set_params(relax="block_jacobi", rebuild_interval=args.amg_rebuild)
+            P = core.freestream_primitive(core.wing_freestream(mach[q], aoa[q]), 3)
+            c.set_freestream(P)
+            c.set_state(np.tile(core.conserved_from_primitive(P, 3), (md.ncell, 1)))
+            c.set_limiter_freeze(freeze)
+            c.synchronize()
+            t = time.perf_counter()
+            h = c.implicit_run(args.max_steps, cmin, cmax, ex, stop_drop=args.drop, repeat_nv=False)
+            c.synchronize()
+            wall = time.perf_counter() - t
+            wall_all += wall
+            cl, cd = mg.force_coefficients(geo["S"], centers, geo["btype"], geo["owner"], c.get_state(), mach[q],
+                                           aoa[q], md.wing["spec"])[:2]
+            reached = bool(h[-1, 5] <= args.drop * h[0, 5])
+            runs.append(dict(mach=float(mach[q]), alpha_deg=float(aoa[q]), steps=len(h), reached=reached,
+                             wall_s=wall, CL=float(cl), CD=float(cd)))
+            log(f"  amg condition {q}: M {mach[q]:.3f} alpha {aoa[q]:.2f}, {len(h)} steps, {wall:.2f}s, "
+                f"reached {reached}")
+    return {"wall_s": wall_all, "conditions_reached": sum(r["reached"] for r in runs),
+            "s_per_condition": wall_all / len(mach),
+            "cell_updates_per_s": md.ncell * sum(r["steps"] for r in runs) / wall_all,
+            "scheme": f"FP32 AMG-GMRES({args.restart}) rtol {args.amg_rtol}, hierarchy rebuilt every "
+                      f"{args.amg_rebuild} solves, CFL {cmin}->{cmax}, limiter frozen at a {freeze:g} drop",
+            "runs": runs}
 
 
 def implicit_bump_workload(args):
@@ -822,6 +867,8 @@ def main():
     ap.add_argument("--sweep-conditions", type=int, default=64, help="conditions of the implicit sweep")
     ap.add_argument("--isweep-batch", type=int, default=16, help="conditions per batch of the implicit sweep")
     ap.add_argument("--sweep-cfl-max", type=float, default=200.0, help="CFL cap of the implicit sweep")
+    ap.add_argument("--isweep-amg", action="store_true",
+                    help="implicit sweep: also run the conditions one at a time with the AMG form")
     ap.add_argument("--sweep-wing", type=int, nargs=3, default=[60, 30, 46])
     ap.add_argument("--graph", action="store_true", help="replay each pseudo-time step as a HIP graph")
     ap.add_argument("--wing", type=int, nargs=3, default=[96, 48, 72])
