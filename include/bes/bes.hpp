@@ -143,6 +143,7 @@ class Config {
   bool getAmgAggregation() const { return amgAggregation_; }
   int getAmgRelax() const { return amgRelax_; }   // 0 damped Jacobi, 1 SPAI0, 2 block Jacobi, -1 other
   int getBlockRepeat() const { return blockRepeat_; }
+  double getLimiterFreezeDrop() const { return limiterFreeze_; }
 
  private:
   void setDefaultValues_() {   // Config.cpp:73-114
@@ -208,6 +209,7 @@ class Config {
     else if (k == "AMG_PRECOND_SMOOTHER")
       amgRelax_ = a[0] == "DAMPED_JACOBI" ? 0 : a[0] == "SPAI0" ? 1 : a[0] == "BLOCK_JACOBI" ? 2 : -1;
     else if (k == "JACOBIAN_BLOCK_REPEAT") blockRepeat_ = a[0] == "NO" ? 0 : 1;
+    else if (k == "LIMITER_FREEZE_DROP") limiterFreeze_ = num(a[0]);   // 0: never (bes_set_limiter_freeze)
     else throw Error(BES_ERR_ARG, "unknown config key " + k);
   }
   std::string meshFile_, outputFolder_ = ".";
@@ -223,6 +225,7 @@ class Config {
   double linTol_ = 1e-7, venkat_ = 5.0, harten_ = 0.05, R_ = 287, gamma_ = 1.4;
   int nDim_ = 0, nConserved_ = 0, nPrimitive_ = 0, nCurrentSolution_ = 1;
   int geomMode_ = BES_GEOM_COMPAT, precision_ = 64, precond_ = 2, blockRepeat_ = 1;
+  double limiterFreeze_ = 0.0;
   bool amgAggregation_ = true;
   int amgRelax_ = 0;
 };
@@ -646,6 +649,7 @@ class SolverImplicit : public Solver {
     check(bes_implicit_setup(mesh->ctx(), config->getPrecision(), config->getPreconditioner(), 30,
                              config->getLinearSolverNumIter(), config->getLinearSolverTol()));
     setupPreconditioner_(mesh->ctx(), config);
+    check(bes_set_limiter_freeze(mesh->ctx(), config->getLimiterFreezeDrop()));
   }
   void setUniversalInitialCondition(double *primitive) override { uniformState_(primitive); }
   // SolverImplicit.cpp:552-600 with the adaptive CFL of :530-550

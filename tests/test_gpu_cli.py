@@ -14,7 +14,7 @@ import pytest
 import scipy.sparse as sp
 import scipy.sparse.linalg as spla
 
-from bes_amd import meshgen as mg
+from bes_amd import core, meshgen as mg
 from oracle import oracle as orc
 
 from test_gpu_output import dimensional, read_csv, read_vtk
@@ -119,3 +119,28 @@ def test_cli_rejects_unknown_key(tmp_path):
     res = _run(tmp_path, md, "EXPLICIT", 1, extra=["NOT_A_KEY = 3"])
     assert res.returncode == 1
     assert "unknown config key" in res.stderr
+
+
+def test_cli_limiter_freeze_key(tmp_path):
+    """LIMITER_FREEZE_DROP (a key of this build) reaches bes_set_limiter_freeze:
+    the driver's residual history is the Python binding's with the same freeze."""
+    md = mg.bump_channel(24, 12, seed=10)
+    n, drop = 8, 0.5
+    res = _run(tmp_path, md, "IMPLICIT", n, extra=["LINEAR_SOLVER_ABS_TOL = 1E-10", "LINEAR_SOLVER_MAX_ITER = 2000",
+                                                   f"LIMITER_FREEZE_DROP = {drop}"])
+    assert res.returncode == 0, res.stderr
+    got = _csv(tmp_path / "residual.csv")
+    finf = orc.freestream_primitive((0.5, 0.05), 2)
+    U0 = np.tile(orc.conserved_from_primitive(finf, 2), (md.ncell, 1))
+    hist = {}
+    for fr in (drop, 0.0):
+        with core.Context(md) as c:
+            c.set_freestream(finf)
+            c.set_state(U0)
+            c.implicit_setup(64, "ilu0", maxiter=2000, rtol=1e-10)
+            c.set_limiter_freeze(fr)
+            hist[fr] = c.implicit_run(n, 5.0, 50.0, (1.1, 0.75), repeat_nv=True)
+            if fr > 0:
+                assert c.limiter_frozen().all()
+    np.testing.assert_allclose(got[:, 1:5], hist[drop][:, 0:4], rtol=1e-6)
+    assert not np.array_equal(hist[drop], hist[0.0])
