@@ -21,6 +21,7 @@
 #include <cstring>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/bes_hip.h"
@@ -119,38 +120,26 @@ __global__ void k_decode_min(const unsigned long long *bits, double *out, int n)
   if (q < n) out[q] = __longlong_as_double((long long)bits[q]);
 }
 
-// reference LimiterVenkatakrishnan.cpp:101-128
+// reference LimiterVenkatakrishnan.cpp:101-128.  The two branches there are one
+// formula in the bound on the side d2 points to (max for d2 > 0, min for
+// d2 < 0); selecting that bound first evaluates the same operations once per
+// lane, so a wave whose cells have both signs (almost every wave) no longer
+// runs the two correctly rounded divisions twice.  d2 == 0 or NaN gives 1.
 __device__ __forceinline__ double face_limiter(double mn, double mx, double d2, double eps2) {
-  if (d2 > 0)
-    return (1 / d2) * ((mx * mx + eps2) * d2 + 2.0 * d2 * d2 * mx) /
-           (mx * mx + 2.0 * d2 * d2 + mx * d2 + eps2);
-  else if (d2 < 0)
-    return (1 / d2) * ((mn * mn + eps2) * d2 + 2.0 * d2 * d2 * mn) /
-           (mn * mn + 2.0 * d2 * d2 + mn * d2 + eps2);
-  return 1.0;
+  const double m = d2 > 0 ? mx : mn;
+  const double r = (1 / d2) * ((m * m + eps2) * d2 + 2.0 * d2 * d2 * m) / (m * m + 2.0 * d2 * d2 + m * d2 + eps2);
+  return (d2 > 0 || d2 < 0) ? r : 1.0;
 }
 
-// Green-Gauss gradient (GradientGreenGauss.cpp:21-135) fused with the
-// Venkatakrishnan limiter (LimiterVenkatakrishnan.cpp:21-99).  One thread per
-// (cell, condition) computes every variable; all of its loads (geometry, the
-// cell's and its neighbours' states) are issued before the arithmetic that
-// needs them.  Every variable's arithmetic is the reference's, in its order.
-// Writes the cell pack, per variable the group [U_v | phi_v * grad U_v] at
-// ((i*NV + v)*GW + q)*B + b, GW = 1 + nd.  (One wave per variable instead
-// keeps fewer registers per thread but repeats the geometry loads and address
-// arithmetic per variable: 427 against 355 us on the 2M-tet box.)
-// With frz non-null (limiter freeze on): a condition b with fmask.c[b] != 0
-// takes its limiter from frz, every other condition stores its limiter there.
-template <int ND>
-__global__ __launch_bounds__(256) void k_gradlim(const double *__restrict__ U, DevMesh dm,
-                                                    double *__restrict__ PK, double *__restrict__ Gout,
-                                                    double *__restrict__ Pout, double *__restrict__ frz,
-                                                    CoefB fmask, int lgB) {
+// the gradient, limiter and pack of one (cell, condition); the pack goes to
+// stage (PW doubles) when STAGE, else to PK
+template <int ND, bool STAGE>
+__device__ __forceinline__ void gradlim_cell(const double *__restrict__ U, const DevMesh &dm,
+                                             double *__restrict__ PK, double *__restrict__ Gout,
+                                             double *__restrict__ Pout, double *__restrict__ frz,
+                                             const CoefB &fmask, int i, int b, int B, double *stage) {
   constexpr int NV = ND + 2, K = ND + 1, GW = ND + 1;
-  const int N = dm.N, B = 1 << lgB;
-  const int t = cell_index();
-  const int i = t >> lgB, b = t & (B - 1);
-  if (i >= N) return;
+  const int N = dm.N;
   int nb[K];
   double S[K][ND], w[K], u[NV], uj[K][NV];
 #pragma unroll
@@ -244,10 +233,9 @@ __global__ __launch_bounds__(256) void k_gradlim(const double *__restrict__ U, D
       pk[1 + d] = best * g[v][d];
     }
     if (Pout) Pout[pq] = best;
-    if (B == 1 && GW % 2 == 0) {
-      double2 *dst = reinterpret_cast<double2 *>(PK + ((size_t)i * NV + v) * GW);
+    if constexpr (STAGE) {
 #pragma unroll
-      for (int q = 0; q < GW / 2; q++) dst[q] = make_double2(pk[2 * q], pk[2 * q + 1]);
+      for (int q = 0; q < GW; q++) stage[v * GW + q] = pk[q];
     } else {
 #pragma unroll
       for (int q = 0; q < GW; q++) PK[(((size_t)i * NV + v) * GW + q) * B + b] = pk[q];
@@ -255,13 +243,51 @@ __global__ __launch_bounds__(256) void k_gradlim(const double *__restrict__ U, D
   }
 }
 
-// reconstructed face state U + (phi*grad).r of one side, from a cell pack
-template <int ND>
-__device__ __forceinline__ void face_state(const double *__restrict__ PK, int cell, int b, int B,
-                                           const double *r, double *uf) {
+// Green-Gauss gradient (GradientGreenGauss.cpp:21-135) fused with the
+// Venkatakrishnan limiter (LimiterVenkatakrishnan.cpp:21-99).  One thread per
+// (cell, condition) computes every variable; all of its loads (geometry, the
+// cell's and its neighbours' states) are issued before the arithmetic that
+// needs them.  Every variable's arithmetic is the reference's, in its order.
+// Writes the cell pack, per variable the group [U_v | phi_v * grad U_v] at
+// ((i*NV + v)*GW + q)*B + b, GW = 1 + nd.  (One wave per variable instead
+// keeps fewer registers per thread but repeats the geometry loads and address
+// arithmetic per variable: 427 against 355 us on the 2M-tet box.)
+// With frz non-null (limiter freeze on): a condition b with fmask.c[b] != 0
+// takes its limiter from frz, every other condition stores its limiter there.
+// With one condition the packs of the block's cells (contiguous in PK) are
+// staged in LDS (blockDim.x * PW doubles of dynamic shared memory) and written
+// out by the block in 16-byte pieces that cover whole cache lines; stored
+// directly, each store instruction of a wave would touch 64 lines 160 bytes
+// apart and lines leave L2 partly written.
+template <int ND, bool ONE>
+__global__ __launch_bounds__(256) void k_gradlim(const double *__restrict__ U, DevMesh dm,
+                                                    double *__restrict__ PK, double *__restrict__ Gout,
+                                                    double *__restrict__ Pout, double *__restrict__ frz,
+                                                    CoefB fmask, int lgB) {
   constexpr int NV = ND + 2, GW = ND + 1, PW = NV * GW;
-  double pk[PW];
-  if (B == 1) {
+  extern __shared__ double sm_pack[];
+  const int N = dm.N;
+  const int i0 = xcd_block() * blockDim.x;
+  const int t = i0 + threadIdx.x;
+  if constexpr (!ONE) {
+    const int B = 1 << lgB, i = t >> lgB, b = t & (B - 1);
+    if (i < N) gradlim_cell<ND, false>(U, dm, PK, Gout, Pout, frz, fmask, i, b, B, nullptr);
+  } else {
+    if (t < N) gradlim_cell<ND, true>(U, dm, PK, Gout, Pout, frz, fmask, t, 0, 1, sm_pack + threadIdx.x * PW);
+    __syncthreads();
+    const int nc = min((int)blockDim.x, N - i0);   // cells of this block (i0 < N: the grid covers N)
+    const double2 *src = reinterpret_cast<const double2 *>(sm_pack);
+    double2 *dst = reinterpret_cast<double2 *>(PK + (size_t)i0 * PW);
+    for (int q = threadIdx.x; q < nc * (PW / 2); q += blockDim.x) dst[q] = src[q];
+  }
+}
+
+// a cell pack [NV][1 + ND] of one condition into registers (ONE: the
+// single-condition layout, the pack in 16-byte pieces)
+template <int ND, bool ONE>
+__device__ __forceinline__ void load_pack(const double *__restrict__ PK, int cell, int b, int B, double *pk) {
+  constexpr int PW = (ND + 2) * (ND + 1);
+  if constexpr (ONE) {
     const double2 *src = reinterpret_cast<const double2 *>(PK + (size_t)cell * PW);
 #pragma unroll
     for (int q = 0; q < PW / 2; q++) {
@@ -273,6 +299,12 @@ __device__ __forceinline__ void face_state(const double *__restrict__ PK, int ce
 #pragma unroll
     for (int q = 0; q < PW; q++) pk[q] = PK[((size_t)cell * PW + q) * B + b];
   }
+}
+
+// reconstructed face state U + (phi*grad).r of one side, from its cell pack
+template <int ND>
+__device__ __forceinline__ void face_state(const double *pk, const double *r, double *uf) {
+  constexpr int NV = ND + 2, GW = ND + 1;
 #pragma unroll
   for (int v = 0; v < NV; v++) {
     uf[v] = pk[v * GW];
@@ -287,28 +319,33 @@ __device__ __forceinline__ void face_state(const double *__restrict__ PK, int ce
 // computed once per face and stored as a record of fs values (nv, padded to an
 // even count when B = 1 so the record moves in 16-byte pieces).  finf:
 // freestream primitive state per condition.
-template <int ND>
+template <int ND, bool ONE>
 __global__ __launch_bounds__(256) void k_face_flux(const double *__restrict__ PK, DevMesh dm, GasConst gc,
                                                    const double *__restrict__ finf, double *__restrict__ FF,
                                                    int fs, unsigned long long *__restrict__ err, int lgB) {
   constexpr int NV = ND + 2;
-  const int F = dm.F, B = 1 << lgB;
+  const int F = dm.F, B = ONE ? 1 : 1 << lgB;
   const int t = xcd_block() * blockDim.x + threadIdx.x;
-  const int f = t >> lgB, b = t & (B - 1);
+  const int f = ONE ? t : t >> lgB, b = ONE ? 0 : t & (B - 1);
   if (f >= F) return;
   const int o = dm.fo[f], j = dm.fnb[f];
-  double nn[ND], rs[ND], ul[NV], out[NV + 1];
+  // both sides' packs and the face geometry are requested before any of them
+  // is used (a boundary face reads its owner's pack twice, from cache), so the
+  // two gathers overlap instead of taking one memory round trip each
+  double nn[ND], rs[ND], rn[ND], ul[NV], out[NV + 1];
+  double pko[(ND + 2) * (ND + 1)], pkn[(ND + 2) * (ND + 1)];
 #pragma unroll
   for (int d = 0; d < ND; d++) {
     nn[d] = dm.fnr[(size_t)d * F + f];
     rs[d] = dm.frs[(size_t)d * F + f];
+    rn[d] = dm.frn[(size_t)d * F + f];
   }
-  face_state<ND>(PK, o, b, B, rs, ul);
+  load_pack<ND, ONE>(PK, o, b, B, pko);
+  load_pack<ND, ONE>(PK, j >= 0 ? j : o, b, B, pkn);
+  face_state<ND>(pko, rs, ul);
   if (j >= 0) {
-    double rn[ND], ur[NV];
-#pragma unroll
-    for (int d = 0; d < ND; d++) rn[d] = dm.frn[(size_t)d * F + f];
-    face_state<ND>(PK, j, b, B, rn, ur);
+    double ur[NV];
+    face_state<ND>(pkn, rn, ur);
     roe_flux<ND, double>(gc, ul, ur, nn, out);
   } else if (j == -1) {
     slip_flux<ND, double>(gc, ul, nn, out);
@@ -323,8 +360,8 @@ __global__ __launch_bounds__(256) void k_face_flux(const double *__restrict__ PK
   for (int v = 0; v < NV; v++) bad |= (out[v] != out[v]);
   if (bad) atomicMin(err, (unsigned long long)f);
   out[NV] = 0.0;
-  if (B == 1 && fs % 2 == 0) {
-    double2 *dst = reinterpret_cast<double2 *>(FF + (size_t)f * fs);
+  if constexpr (ONE) {   // fs = NV rounded up to even
+    double2 *dst = reinterpret_cast<double2 *>(FF + (size_t)f * ((NV + 1) / 2 * 2));
 #pragma unroll
     for (int q = 0; q < (NV + 1) / 2; q++) dst[q] = make_double2(out[2 * q], out[2 * q + 1]);
   } else {
@@ -338,7 +375,7 @@ __global__ __launch_bounds__(256) void k_face_flux(const double *__restrict__ PK
 // stage update U = U0 - alpha*dt/V*R (:364-375) and the Linf monitor (:356-362).
 // The faces come pre-sorted in that order with the area negated where the cell
 // is the neighbour (R + x*(-A) is R - x*A exactly).
-template <int ND>
+template <int ND, bool ONE>
 __global__ __launch_bounds__(256) void k_cell_residual(
     // U0 and Ud may be the same array (the last stage updates in place: each
     // thread reads only its own U0 entry before writing it), so no restrict
@@ -346,9 +383,9 @@ __global__ __launch_bounds__(256) void k_cell_residual(
     const double *__restrict__ dtg, double alpha, DevMesh dm, double *__restrict__ Rout,
     unsigned long long *__restrict__ linf_bits, int lgB) {
   constexpr int NV = ND + 2, K = ND + 1;
-  const int N = dm.N, B = 1 << lgB;
+  const int N = dm.N, B = ONE ? 1 : 1 << lgB;
   const int t = cell_index();
-  const int i = t >> lgB, b = t & (B - 1);
+  const int i = ONE ? t : t >> lgB, b = ONE ? 0 : t & (B - 1);
   double R[NV];
 #pragma unroll
   for (int v = 0; v < NV; v++) R[v] = 0.0;
@@ -363,8 +400,8 @@ __global__ __launch_bounds__(256) void k_cell_residual(
 #pragma unroll
     for (int q = 0; q < K; q++) {
       double x[NV + 1];
-      if (B == 1 && fs % 2 == 0) {
-        const double2 *src = reinterpret_cast<const double2 *>(FF + (size_t)fq[q] * fs);
+      if constexpr (ONE) {   // fs = NV rounded up to even
+        const double2 *src = reinterpret_cast<const double2 *>(FF + (size_t)fq[q] * ((NV + 1) / 2 * 2));
 #pragma unroll
         for (int p = 0; p < (NV + 1) / 2; p++) {
           const double2 tt = src[p];
@@ -702,6 +739,21 @@ int launch_timestep(bes_ctx *c, const double *U, double cfl, int local, const do
   return BES_OK;
 }
 
+// calls f(integral_constant nd, bool_constant one) for the kernel instance
+// of the mesh dimension and of a single-condition (one) or batched launch
+template <typename Fn>
+void by_shape(int nd, bool one, Fn &&f) {
+  using I2 = std::integral_constant<int, 2>;
+  using I3 = std::integral_constant<int, 3>;
+  if (nd == 2) {
+    if (one) f(I2{}, std::true_type{});
+    else f(I2{}, std::false_type{});
+  } else {
+    if (one) f(I3{}, std::true_type{});
+    else f(I3{}, std::false_type{});
+  }
+}
+
 // algorithmic HBM bytes of one gradient+limiter launch: state, geometry, pack
 double gradlim_bytes(const bes_ctx *c) {
   const double N = c->N, B = c->B, nv = c->nv, nd = c->nd, k = c->k;
@@ -716,12 +768,13 @@ int launch_gradlim(bes_ctx *c, const double *U, double *Gout, double *Pout) {
   for (int b = 0; b < bes_ctx::kMaxBatch; b++)
     fm.c[b] = (b < (int)c->lim_frozen.size() && c->lim_frozen[b]) ? 1.0 : 0.0;
   double *frz = c->lim_freeze_on ? c->phi_frz : nullptr;
-  if (c->nd == 2)
-    hipLaunchKernelGGL(k_gradlim<2>, dim3(nb), dim3(bs), 0, c->stream, U, c->dm, c->LG, Gout, Pout, frz, fm,
-                       c->lgB);
-  else
-    hipLaunchKernelGGL(k_gradlim<3>, dim3(nb), dim3(bs), 0, c->stream, U, c->dm, c->LG, Gout, Pout, frz, fm,
-                       c->lgB);
+  // one condition: the block's packs are staged in LDS
+  const bool one = c->B == 1;
+  const size_t lds = one ? (size_t)bs * c->nv * (c->nd + 1) * sizeof(double) : 0;
+  by_shape(c->nd, one, [&](auto nd, auto one_) {
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_gradlim<decltype(nd)::value, decltype(one_)::value>),
+                       dim3(nb), dim3(bs), lds, c->stream, U, c->dm, c->LG, Gout, Pout, frz, fm, c->lgB);
+  });
   BES_HIP(hipGetLastError());
   return BES_OK;
 }
@@ -735,14 +788,13 @@ double faceflux_bytes(const bes_ctx *c) {
 // face fluxes of the current packs (shared with the implicit assembly)
 int launch_face_flux(bes_ctx *c) {
   const int bs = tuned_bs(c, kProfFaceFlux), nb = grid_for(c->dm.F * c->B, bs);
-  const int nd = c->nd;
+  const bool one = c->B == 1;
   ProfScope ps(c, kProfFaceFlux, faceflux_bytes(c));
-  if (nd == 2)
-    hipLaunchKernelGGL(k_face_flux<2>, dim3(nb), dim3(bs), 0, c->stream, c->LG, c->dm, c->gc, c->dfinf, c->FF,
-                       c->fs, c->err, c->lgB);
-  else
-    hipLaunchKernelGGL(k_face_flux<3>, dim3(nb), dim3(bs), 0, c->stream, c->LG, c->dm, c->gc, c->dfinf, c->FF,
-                       c->fs, c->err, c->lgB);
+  by_shape(c->nd, one, [&](auto nd, auto one_) {
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_face_flux<decltype(nd)::value, decltype(one_)::value>),
+                       dim3(nb), dim3(bs), 0, c->stream, c->LG, c->dm, c->gc, c->dfinf, c->FF, c->fs, c->err,
+                       c->lgB);
+  });
   BES_HIP(hipGetLastError());
   return BES_OK;
 }
@@ -762,13 +814,13 @@ int launch_residual(bes_ctx *c, const double *U0, double *Ud, double alpha, int 
   if (s) return s;
   const int bs = tuned_bs(c, kProfResidual), nb = grid_for(c->N * c->B, bs);
   const double *dtg = local ? nullptr : c->dtg;
+  const bool one = c->B == 1;
   ProfScope ps(c, kProfResidual, residual_bytes(c, U0 != nullptr, Rout != nullptr));
-  if (c->nd == 2)
-    hipLaunchKernelGGL(k_cell_residual<2>, dim3(nb), dim3(bs), 0, c->stream, c->FF, c->fs, U0, Ud, c->dt, dtg, alpha,
-                       c->dm, Rout, linf_bits, c->lgB);
-  else
-    hipLaunchKernelGGL(k_cell_residual<3>, dim3(nb), dim3(bs), 0, c->stream, c->FF, c->fs, U0, Ud, c->dt, dtg, alpha,
-                       c->dm, Rout, linf_bits, c->lgB);
+  by_shape(c->nd, one, [&](auto nd, auto one_) {
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cell_residual<decltype(nd)::value, decltype(one_)::value>),
+                       dim3(nb), dim3(bs), 0, c->stream, c->FF, c->fs, U0, Ud, c->dt, dtg, alpha, c->dm, Rout,
+                       linf_bits, c->lgB);
+  });
   BES_HIP(hipGetLastError());
   return BES_OK;
 }
