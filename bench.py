@@ -837,10 +837,26 @@ def copy_probe():
     x = torch.ones(n, dtype=torch.float32, device="cuda")
     y = torch.empty_like(x)
     torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    y.copy_(x)
+    e0.record()
     for _ in range(5):
         y.copy_(x)
+    e1.record()
     torch.cuda.synchronize()
-    print(json.dumps({"copy_bytes": n * 4, "copies": 5}))
+    ms = e0.elapsed_time(e1) / 5
+    s = x.sum()
+    e0.record()
+    for _ in range(5):
+        s = x.sum()
+    e1.record()
+    torch.cuda.synchronize()
+    ms_read = e0.elapsed_time(e1) / 5
+    # each copy reads and writes n*4 bytes, each sum reads them: the stream
+    # rates a kernel can reach in practice
+    print(json.dumps({"copy_bytes": n * 4, "copies": 6, "ms_per_copy": ms,
+                      "GBps_read_plus_write": 2 * n * 4 / (ms * 1e-3) / 1e9,
+                      "ms_per_sum": ms_read, "GBps_read": n * 4 / (ms_read * 1e-3) / 1e9}))
 
 
 def traffic_from_profiles(kernel):
@@ -883,7 +899,7 @@ def main():
     ap.add_argument("--repeat", dest="no_repeat", action="store_false")
     ap.add_argument("--quick-drop", action="store_true", help="skip the mixed FP32 ILU(0) drop run")
     ap.add_argument("--copy-probe", action="store_true",
-                    help="only copy 1 GiB device to device 5 times (calibrates HBM byte counters)")
+                    help="only copy 1 GiB device to device 6 times (calibrates HBM byte counters; prints the copy rate)")
     ap.add_argument("--timed-form", choices=sorted(TIMED_FORMS), default="plain",
                     help="form of the K timed implicit steps (the headline value is the plain form's)")
     ap.add_argument("--lp-rebuild", type=int, default=6,

@@ -197,6 +197,7 @@ __device__ __forceinline__ void gradlim_cell(const double *__restrict__ U, const
     }
   }
   const double V = dm.vol[i];
+  const RcpDiv divV(V);   // the gradient's nv*nd divisions by V, as one
   const double eps2 = dm.eps2[i];
   double rl[K][ND];
 #pragma unroll
@@ -206,7 +207,7 @@ __device__ __forceinline__ void gradlim_cell(const double *__restrict__ U, const
 #pragma unroll
   for (int v = 0; v < NV; v++) {
 #pragma unroll
-    for (int d = 0; d < ND; d++) g[v][d] /= V;
+    for (int d = 0; d < ND; d++) g[v][d] = divV(g[v][d]);
     double best = 1.0;
     const size_t pq = ((size_t)v * N + i) * B + b;
     if (frz && fmask.c[b] != 0.0) {
@@ -267,7 +268,7 @@ __global__ __launch_bounds__(256) void k_gradlim(const double *__restrict__ U, D
   constexpr int NV = ND + 2, GW = ND + 1, PW = NV * GW;
   extern __shared__ double sm_pack[];
   const int N = dm.N;
-  const int i0 = xcd_block() * blockDim.x;
+  const int i0 = blockIdx.x * blockDim.x;   // dispatch order (see k_cell_residual)
   const int t = i0 + threadIdx.x;
   if constexpr (!ONE) {
     const int B = 1 << lgB, i = t >> lgB, b = t & (B - 1);
@@ -325,7 +326,7 @@ __global__ __launch_bounds__(256) void k_face_flux(const double *__restrict__ PK
                                                    int fs, unsigned long long *__restrict__ err, int lgB) {
   constexpr int NV = ND + 2;
   const int F = dm.F, B = ONE ? 1 : 1 << lgB;
-  const int t = xcd_block() * blockDim.x + threadIdx.x;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;   // dispatch order (see k_cell_residual)
   const int f = ONE ? t : t >> lgB, b = ONE ? 0 : t & (B - 1);
   if (f >= F) return;
   const int o = dm.fo[f], j = dm.fnb[f];
@@ -374,7 +375,11 @@ __global__ __launch_bounds__(256) void k_face_flux(const double *__restrict__ PK
 // in the reference face order (SolverExplicit.cpp:325-341), fused with the
 // stage update U = U0 - alpha*dt/V*R (:364-375) and the Linf monitor (:356-362).
 // The faces come pre-sorted in that order with the area negated where the cell
-// is the neighbour (R + x*(-A) is R - x*A exactly).
+// is the neighbour (R + x*(-A) is R - x*A exactly).  Blocks run in dispatch
+// order (consecutive blocks on different XCDs): on the 2M-tet box this kernel,
+// the face flux and the gradient took 4%, 2% and 1% less time than with the
+// XCD-contiguous block map of cell_index(), whose shared-L2 reuse of
+// neighbour data does not pay for these gathers.
 template <int ND, bool ONE>
 __global__ __launch_bounds__(256) void k_cell_residual(
     // U0 and Ud may be the same array (the last stage updates in place: each
@@ -384,7 +389,7 @@ __global__ __launch_bounds__(256) void k_cell_residual(
     unsigned long long *__restrict__ linf_bits, int lgB) {
   constexpr int NV = ND + 2, K = ND + 1;
   const int N = dm.N, B = ONE ? 1 : 1 << lgB;
-  const int t = cell_index();
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int i = ONE ? t : t >> lgB, b = ONE ? 0 : t & (B - 1);
   double R[NV];
 #pragma unroll

@@ -76,7 +76,7 @@ __global__ __launch_bounds__(256) void k_face_jacobian(const double *__restrict_
   // records; a face's two blocks sit side by side in one record [2][JP] (JL,
   // then JR, row-major), entry e of condition b at (rec + e)*B + b
   const int B = 1 << lgB;
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = xcd_block() * blockDim.x + threadIdx.x;
   const int grp = t / (64 * 2 * NV), rem = t - grp * (64 * 2 * NV);
   const int dir = rem >> 6, l = rem & 63;
   const int f = grp * (64 >> lgB) + (l >> lgB), b = l & (B - 1);
@@ -156,7 +156,13 @@ __global__ __launch_bounds__(256) void k_assemble(
     }
 #pragma unroll
     for (int v = 0; v < NV; v++) b[((size_t)i * NV + v) * B + bq] = bi[v];
-    // blocks: identity, then per face (reference order) the repeated adds
+    // blocks: identity, then per face (reference order) the repeated adds.
+    // own row: diag += mul*JL, off(i,nbr) = mul*JR; neighbour row:
+    // diag += (-m)*JR, off(i,owner) = (-m)*JL.  The diagonal sums in the
+    // reference's face order; each off-diagonal block depends on its own face
+    // only, so those are written in a second loop in slot order, where every
+    // lane of a wave stores into the same slot's planes (whole cache lines;
+    // in face order the lanes' slots differ and lines leave L2 part-written).
     double D[NB];
 #pragma unroll
     for (int e = 0; e < NB; e++) D[e] = 0.0;
@@ -165,12 +171,9 @@ __global__ __launch_bounds__(256) void k_assemble(
     for (int q = 0; q < K; q++) {
       const int m = (info >> (2 * q)) & 3;
       const bool own = (info >> (8 + m)) & 1;
-      const int j = dm.nbr[(size_t)m * N + i];
       const int f = fd.cf[(size_t)m * N + i];
       const double Af = dm.af[(size_t)m * N + i];
       const double mul = own ? mdt * Af : -(mdt * Af);
-      // own row: diag += mul*JL, off(i,nbr) = mul*JR; neighbour row:
-      // diag += (-m)*JR, off(i,owner) = (-m)*JL
       double J[NB];
       const TM *Jd = JF + ((size_t)f * 2 + (own ? 0 : 1)) * JP * B;
       if (B == 1) load_rec<TM, NB>(Jd, J);
@@ -183,6 +186,15 @@ __global__ __launch_bounds__(256) void k_assemble(
         const double x = mul * J[e];
         for (int r = 0; r < repeat; r++) D[e] += x;
       }
+    }
+#pragma unroll
+    for (int m = 0; m < K; m++) {
+      const bool own = (info >> (8 + m)) & 1;
+      const int j = dm.nbr[(size_t)m * N + i];
+      const int f = fd.cf[(size_t)m * N + i];
+      const double Af = dm.af[(size_t)m * N + i];
+      const double mul = own ? mdt * Af : -(mdt * Af);
+      double J[NB];
       TM *Ao = A + (size_t)(1 + m) * NB * N * B;
       if (j >= 0) {
         const TM *Jo = JF + ((size_t)f * 2 + (own ? 1 : 0)) * JP * B;

@@ -18,6 +18,27 @@ __device__ __forceinline__ int xcd_block() {
 
 __device__ __forceinline__ int cell_index() { return xcd_block() * blockDim.x + threadIdx.x; }
 
+// Quotients a / b for one divisor b, bitwise equal to the correctly rounded
+// division, with one division in all.  y = RN(1/b) is computed once; then
+// q = RN(a*y) is within an ulp of a/b, its remainder a - q*b is exact in one
+// FMA, and RN(q + r*y) is the correctly rounded quotient (Markstein's theorem,
+// which needs no underflow or overflow on the way: hence the ranges, outside
+// of which the plain division is taken).
+struct RcpDiv {
+  double b, y;
+  bool ok;
+  __host__ __device__ explicit RcpDiv(double b_)
+      : b(b_), y(1.0 / b_), ok(__builtin_fabs(b_) >= 0x1p-500 && __builtin_fabs(b_) <= 0x1p+500) {}
+  __host__ __device__ double operator()(double a) const {
+    const double aa = __builtin_fabs(a);
+    if (ok && aa >= 0x1p-400 && aa <= 0x1p+400) {
+      const double q = a * y;
+      return __builtin_fma(__builtin_fma(-q, b, a), y, q);
+    }
+    return a / b;
+  }
+};
+
 // one value per condition of a batch (at most 64), passed to a kernel by value
 struct CoefB {
   double c[64];
