@@ -451,18 +451,34 @@ def config1_explicit(args):
         c.synchronize()
         wall_gpu = time.perf_counter() - t
         Ug = c.get_state()
+        # the persistent form: all steps in one cooperative launch
+        c.explicit_graph(False)
+        c.explicit_persistent(True)
+        c.explicit_run(2, 0.8)
+        c.set_state(U0)
+        c.synchronize()
+        t = time.perf_counter()
+        lp = c.explicit_run(niter, 0.8)
+        c.synchronize()
+        wall_pers = time.perf_counter() - t
+        same = bool(np.array_equal(c.get_state(), Ug) and np.array_equal(lp, lg))
+        c.explicit_persistent(False)
     threads = omp_threads()
     m = orc.Mesh(md)
     t = time.perf_counter()
     Uc, lc = m.explicit(U0, niter, 0.8, finf, nthreads=threads)
     wall_cpu = time.perf_counter() - t
-    log(f"config 1: GPU {wall_gpu:.2f}s, CPU {wall_cpu:.2f}s for {niter} iterations")
+    log(f"config 1: GPU {wall_gpu:.2f}s (graph), {wall_pers:.2f}s (persistent), CPU {wall_cpu:.2f}s "
+        f"for {niter} iterations")
     return {"workload": "explicit-bump-40k", "cells": md.ncell, "iterations": niter,
             "scheme": "RK4 (4-stage), Roe, Green-Gauss, Venkatakrishnan K=5, local dt, CFL 0.8, FP64",
             "gpu": {"wall_s": wall_gpu, "cell_updates_per_s": md.ncell * niter / wall_gpu,
                     "launch": "one HIP graph per pseudo-time step",
                     "density_linf_drop": float(lg[-1, 0, 0] / lg[0, 0, 0]) if lg.ndim == 3 else
                     float(lg[-1, 0] / lg[0, 0])},
+            "gpu_persistent": {"wall_s": wall_pers, "cell_updates_per_s": md.ncell * niter / wall_pers,
+                               "launch": "one cooperative launch for all iterations",
+                               "bitwise_equal_to_graph": same},
             "cpu": {"wall_s": wall_cpu, "cell_updates_per_s": md.ncell * niter / wall_cpu, "cores": threads,
                     "kind": "port", "density_linf_drop": float(lc[-1, 0] / lc[0, 0])},
             "gpu_over_cpu": wall_cpu / wall_gpu,

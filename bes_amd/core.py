@@ -125,6 +125,7 @@ OPTIONAL_SIGS = {
     "bes_get_batch": [C.c_void_p],
     "bes_set_freestream_batch": [C.c_void_p, C.c_void_p],
     "bes_explicit_graph": [C.c_void_p, C.c_int],
+    "bes_explicit_persistent": [C.c_void_p, C.c_int],
     "bes_lp_param_count": [],
     "bes_lp_init": [C.c_void_p, C.c_ulonglong],
     "bes_lp_set_params": [C.c_void_p, C.c_void_p, C.c_int],
@@ -568,6 +569,11 @@ class Context:
     def explicit_graph(self, on=True):
         """Replay each pseudo-time step as one captured HIP graph."""
         _check(lib().bes_explicit_graph(self.h, int(on)))
+
+    def explicit_persistent(self, on=True):
+        """Run all steps of explicit_run in one cooperative persistent launch
+        (one condition, local time steps)."""
+        _check(lib().bes_explicit_persistent(self.h, int(on)))
 
     def explicit_run(self, niter, cfl, local=True, alpha=RK4_ALPHA, monitor=True):
         a = np.ascontiguousarray(alpha, np.float64)

@@ -95,6 +95,8 @@ struct bes_ctx {
   int linf_cap = 0;
   // pseudo-time step captured as a HIP graph (bes_explicit_graph)
   bool use_graph = false;
+  // all steps of a run in one cooperative launch (bes_explicit_persistent)
+  bool use_persistent = false;
   hipGraphExec_t graph = nullptr;
   int graph_key[5] = {0, 0, 0, 0, 0};  // nstage, local, linf on, B, limiter-freeze epoch of the capture
   double graph_cfl = 0.0;

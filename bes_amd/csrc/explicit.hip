@@ -13,6 +13,7 @@
 // One thread per cell; every cell computes the fluxes of its own faces, so no
 // atomics touch the residual and the result is deterministic.  Blocks are
 // remapped so that each XCD walks a contiguous range of Morton-ordered cells.
+#include <hip/hip_cooperative_groups.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -88,27 +89,34 @@ int prof_collect(bes_ctx *c) {
 // for B = 1 the layout is the plain SoA U[v][N].
 
 // cfl_b: per-condition Courant numbers when per_cond is set, else cfl for all
+// local time step of (cell i, condition b), stored in dt and returned
+template <int ND>
+__device__ __forceinline__ double timestep_cell(const double *__restrict__ U, const DevMesh &dm, const GasConst &gc,
+                                                double cfl, double *__restrict__ dt, int i, int b, int B) {
+  constexpr int NV = ND + 2;
+  const int N = dm.N;
+  double u[NV];
+#pragma unroll
+  for (int v = 0; v < NV; v++) u[v] = U[((size_t)v * N + i) * B + b];
+  Gas<ND, double> g(gc);
+  double c = g.sound_speed(u);
+  double sr = 0.0;
+#pragma unroll
+  for (int d = 0; d < ND; d++) sr += (c + ::fabs(g.velocity(u, d))) * dm.sproj[(size_t)d * N + i];
+  const double tv = cfl * dm.vol[i] / sr;
+  dt[(size_t)i * B + b] = tv;
+  return tv;
+}
+
 template <int ND>
 __global__ __launch_bounds__(256) void k_timestep(const double *__restrict__ U, DevMesh dm, GasConst gc,
                                                   double cfl, CoefB cfl_b, int per_cond, double *__restrict__ dt,
                                                   unsigned long long *__restrict__ gmin, int lgB) {
-  constexpr int NV = ND + 2;
   const int N = dm.N, B = 1 << lgB;
   const int t = cell_index();
   const int i = t >> lgB, b = t & (B - 1);
   double tv = __builtin_inf();
-  if (i < N) {
-    double u[NV];
-#pragma unroll
-    for (int v = 0; v < NV; v++) u[v] = U[((size_t)v * N + i) * B + b];
-    Gas<ND, double> g(gc);
-    double c = g.sound_speed(u);
-    double sr = 0.0;
-#pragma unroll
-    for (int d = 0; d < ND; d++) sr += (c + ::fabs(g.velocity(u, d))) * dm.sproj[(size_t)d * N + i];
-    tv = (per_cond ? cfl_b.c[b] : cfl) * dm.vol[i] / sr;
-    dt[(size_t)i * B + b] = tv;
-  }
+  if (i < N) tv = timestep_cell<ND>(U, dm, gc, per_cond ? cfl_b.c[b] : cfl, dt, i, b, B);
   if (gmin) {
     double m = block_min_cond(tv, B);
     if ((int)threadIdx.x < B) atomicMin(&gmin[threadIdx.x], (unsigned long long)__double_as_longlong(m));
@@ -260,6 +268,24 @@ __device__ __forceinline__ void gradlim_cell(const double *__restrict__ U, const
 // out by the block in 16-byte pieces that cover whole cache lines; stored
 // directly, each store instruction of a wave would touch 64 lines 160 bytes
 // apart and lines leave L2 partly written.
+// the cells of block blk (one condition), packs staged in sm (blockDim.x*PW
+// doubles); every thread of the block must call it
+template <int ND>
+__device__ __forceinline__ void gradlim_block_one(const double *__restrict__ U, const DevMesh &dm,
+                                                  double *__restrict__ PK, double *__restrict__ Gout,
+                                                  double *__restrict__ Pout, double *__restrict__ frz,
+                                                  const CoefB &fmask, int blk, double *sm) {
+  constexpr int PW = (ND + 2) * (ND + 1);
+  const int N = dm.N, i0 = blk * blockDim.x, t = i0 + threadIdx.x;
+  if (t < N) gradlim_cell<ND, true>(U, dm, PK, Gout, Pout, frz, fmask, t, 0, 1, sm + threadIdx.x * PW);
+  __syncthreads();
+  const int nc = min((int)blockDim.x, N - i0);   // cells of this block (i0 < N: the blocks cover N)
+  const double2 *src = reinterpret_cast<const double2 *>(sm);
+  double2 *dst = reinterpret_cast<double2 *>(PK + (size_t)i0 * PW);
+  for (int q = threadIdx.x; q < nc * (PW / 2); q += blockDim.x) dst[q] = src[q];
+  __syncthreads();   // sm is free again for the block's next call
+}
+
 template <int ND, bool ONE>
 __global__ __launch_bounds__(256) void k_gradlim(const double *__restrict__ U, DevMesh dm,
                                                     double *__restrict__ PK, double *__restrict__ Gout,
@@ -274,12 +300,7 @@ __global__ __launch_bounds__(256) void k_gradlim(const double *__restrict__ U, D
     const int B = 1 << lgB, i = t >> lgB, b = t & (B - 1);
     if (i < N) gradlim_cell<ND, false>(U, dm, PK, Gout, Pout, frz, fmask, i, b, B, nullptr);
   } else {
-    if (t < N) gradlim_cell<ND, true>(U, dm, PK, Gout, Pout, frz, fmask, t, 0, 1, sm_pack + threadIdx.x * PW);
-    __syncthreads();
-    const int nc = min((int)blockDim.x, N - i0);   // cells of this block (i0 < N: the grid covers N)
-    const double2 *src = reinterpret_cast<const double2 *>(sm_pack);
-    double2 *dst = reinterpret_cast<double2 *>(PK + (size_t)i0 * PW);
-    for (int q = threadIdx.x; q < nc * (PW / 2); q += blockDim.x) dst[q] = src[q];
+    gradlim_block_one<ND>(U, dm, PK, Gout, Pout, frz, fmask, blockIdx.x, sm_pack);
   }
 }
 
@@ -321,12 +342,11 @@ __device__ __forceinline__ void face_state(const double *pk, const double *r, do
 // even count when B = 1 so the record moves in 16-byte pieces).  finf:
 // freestream primitive state per condition.
 template <int ND, bool ONE>
-__global__ __launch_bounds__(256) void k_face_flux(const double *__restrict__ PK, DevMesh dm, GasConst gc,
-                                                   const double *__restrict__ finf, double *__restrict__ FF,
-                                                   int fs, unsigned long long *__restrict__ err, int lgB) {
+__device__ __forceinline__ void face_flux_thread(const double *__restrict__ PK, const DevMesh &dm, const GasConst &gc,
+                                                 const double *__restrict__ finf, double *__restrict__ FF, int fs,
+                                                 unsigned long long *__restrict__ err, int lgB, int t) {
   constexpr int NV = ND + 2;
   const int F = dm.F, B = ONE ? 1 : 1 << lgB;
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;   // dispatch order (see k_cell_residual)
   const int f = ONE ? t : t >> lgB, b = ONE ? 0 : t & (B - 1);
   if (f >= F) return;
   const int o = dm.fo[f], j = dm.fnb[f];
@@ -371,6 +391,14 @@ __global__ __launch_bounds__(256) void k_face_flux(const double *__restrict__ PK
   }
 }
 
+template <int ND, bool ONE>
+__global__ __launch_bounds__(256) void k_face_flux(const double *__restrict__ PK, DevMesh dm, GasConst gc,
+                                                   const double *__restrict__ finf, double *__restrict__ FF,
+                                                   int fs, unsigned long long *__restrict__ err, int lgB) {
+  // dispatch order (see k_cell_residual)
+  face_flux_thread<ND, ONE>(PK, dm, gc, finf, FF, fs, err, lgB, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
 // One thread per (cell, condition): residual sum_f +-F*A over the cell's faces
 // in the reference face order (SolverExplicit.cpp:325-341), fused with the
 // stage update U = U0 - alpha*dt/V*R (:364-375) and the Linf monitor (:356-362).
@@ -380,16 +408,18 @@ __global__ __launch_bounds__(256) void k_face_flux(const double *__restrict__ PK
 // the face flux and the gradient took 4%, 2% and 1% less time than with the
 // XCD-contiguous block map of cell_index(), whose shared-L2 reuse of
 // neighbour data does not pay for these gathers.
+// the cells of block blk; every thread of the block must call it (the Linf
+// monitor is a block reduction)
 template <int ND, bool ONE>
-__global__ __launch_bounds__(256) void k_cell_residual(
+__device__ __forceinline__ void cell_residual_block(
     // U0 and Ud may be the same array (the last stage updates in place: each
     // thread reads only its own U0 entry before writing it), so no restrict
     const double *__restrict__ FF, int fs, const double *U0, double *Ud, const double *__restrict__ dt,
-    const double *__restrict__ dtg, double alpha, DevMesh dm, double *__restrict__ Rout,
-    unsigned long long *__restrict__ linf_bits, int lgB) {
+    const double *__restrict__ dtg, double alpha, const DevMesh &dm, double *__restrict__ Rout,
+    unsigned long long *__restrict__ linf_bits, int lgB, int blk) {
   constexpr int NV = ND + 2, K = ND + 1;
   const int N = dm.N, B = ONE ? 1 : 1 << lgB;
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = blk * blockDim.x + threadIdx.x;
   const int i = ONE ? t : t >> lgB, b = ONE ? 0 : t & (B - 1);
   double R[NV];
 #pragma unroll
@@ -438,9 +468,19 @@ __global__ __launch_bounds__(256) void k_cell_residual(
     for (int v = 0; v < NV; v++) {
       double a = block_max_cond(::fabs(R[v]), B);
       if ((int)threadIdx.x < B)
-        atomicMax(&linf_slot(linf_bits, NV * B)[v * B + threadIdx.x], (unsigned long long)__double_as_longlong(a));
+        atomicMax(&linf_slot_of(linf_bits, NV * B, blk)[v * B + threadIdx.x],
+                  (unsigned long long)__double_as_longlong(a));
     }
   }
+}
+
+template <int ND, bool ONE>
+__global__ __launch_bounds__(256) void k_cell_residual(const double *__restrict__ FF, int fs, const double *U0,
+                                                       double *Ud, const double *__restrict__ dt,
+                                                       const double *__restrict__ dtg, double alpha, DevMesh dm,
+                                                       double *__restrict__ Rout,
+                                                       unsigned long long *__restrict__ linf_bits, int lgB) {
+  cell_residual_block<ND, ONE>(FF, fs, U0, Ud, dt, dtg, alpha, dm, Rout, linf_bits, lgB, blockIdx.x);
 }
 
 __global__ void k_fill_u64(unsigned long long *p, int n, unsigned long long v) {
@@ -468,6 +508,68 @@ __global__ void k_linf_reduce(unsigned long long *bits, int nq) {
   if (q < nq) {
     const unsigned long long m = linf_slots_max(bits, nq, q);
     bits[q] = m;
+  }
+}
+
+// Persistent form of bes_explicit_run for one condition with local time
+// steps: one cooperative launch (every block resident) runs all niter
+// pseudo-time steps, the passes of each (time step; per stage gradient, face
+// fluxes, residual and update) separated by grid barriers, each pass a loop
+// of the grid's blocks over the pass's blocks of 256 cells or faces.  Every
+// value is computed by the same device code as the per-pass kernels, so the
+// states and monitors are bitwise those of the launched form; what goes is
+// the launch per pass (14 per RK4 step), which dominates on small meshes.
+struct PersistStep {
+  double *U0, *Ua, *Ub;   // state slots as in explicit_iteration
+  double *PK, *FF, *dt, *frz, *linf;
+  const double *finf;
+  unsigned long long *lbits, *err;
+  double cfl, alpha[16];
+  int nstage, niter, linf_cap;
+};
+
+template <int ND>
+__global__ __launch_bounds__(256) void k_explicit_persistent(PersistStep a, DevMesh dm, GasConst gc, CoefB fmask) {
+  constexpr int NV = ND + 2, FS = (NV + 1) / 2 * 2;
+  extern __shared__ double sm_pack[];
+  cooperative_groups::grid_group grid = cooperative_groups::this_grid();
+  const int N = dm.N, nbc = (N + blockDim.x - 1) / blockDim.x, nbf = (dm.F + blockDim.x - 1) / blockDim.x;
+  double *U0 = a.U0, *Ua = a.Ua;
+  for (int it = 0; it < a.niter; it++) {
+    for (int blk = blockIdx.x; blk < nbc; blk += gridDim.x) {
+      const int i = blk * blockDim.x + threadIdx.x;
+      if (i < N) timestep_cell<ND>(U0, dm, gc, a.cfl, a.dt, i, 0, 1);
+    }
+    grid.sync();
+    for (int st = 0; st < a.nstage; st++) {
+      const bool last = st == a.nstage - 1;
+      const double *src = st == 0 ? U0 : ((st - 1) % 2 == 0 ? Ua : a.Ub);
+      double *dst = (last && a.nstage > 1) ? U0 : (st % 2 == 0 ? Ua : a.Ub);
+      for (int blk = blockIdx.x; blk < nbc; blk += gridDim.x)
+        gradlim_block_one<ND>(src, dm, a.PK, nullptr, nullptr, a.frz, fmask, blk, sm_pack);
+      grid.sync();
+      // the monitor words are cleared here: block 0 decoded the previous
+      // step's before it reached the barrier above, and the adds come after
+      // the next one
+      if (st == 0 && a.linf)
+        for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < kLinfSlots * NV; q += gridDim.x * blockDim.x)
+          a.lbits[q] = 0ull;
+      for (int blk = blockIdx.x; blk < nbf; blk += gridDim.x)
+        face_flux_thread<ND, true>(a.PK, dm, gc, a.finf, a.FF, FS, a.err, 0, blk * blockDim.x + threadIdx.x);
+      grid.sync();
+      for (int blk = blockIdx.x; blk < nbc; blk += gridDim.x)
+        cell_residual_block<ND, true>(a.FF, FS, U0, dst, a.dt, nullptr, a.alpha[st], dm, nullptr,
+                                      (last && a.linf) ? a.lbits : nullptr, 0, blk);
+      grid.sync();
+    }
+    if (a.linf && blockIdx.x == 0 && (int)threadIdx.x < NV && it < a.linf_cap)
+      a.linf[(size_t)it * NV + threadIdx.x] =
+          __longlong_as_double((long long)linf_slots_max(a.lbits, NV, threadIdx.x));
+    if (a.nstage == 1) {
+      double *t = U0;
+      U0 = Ua;
+      Ua = t;
+    }
   }
 }
 
@@ -1193,6 +1295,51 @@ static int capture_iteration(bes_ctx *c, int nstage, const double *alpha, double
   return BES_OK;
 }
 
+// every pass of niter pseudo-time steps in one cooperative launch (one
+// condition, local time steps; k_explicit_persistent)
+static int explicit_run_persistent(bes_ctx *c, int niter, int nstage, const double *alpha, double cfl, bool linf) {
+  PersistStep a;
+  a.U0 = c->U0;
+  a.Ua = c->Ua;
+  a.Ub = c->Ub;
+  a.PK = c->LG;
+  a.FF = c->FF;
+  a.dt = c->dt;
+  a.frz = c->lim_freeze_on ? c->phi_frz : nullptr;
+  a.linf = linf ? c->linf : nullptr;
+  a.finf = c->dfinf;
+  a.lbits = c->lbits;
+  a.err = c->err;
+  a.cfl = cfl;
+  for (int q = 0; q < 16; q++) a.alpha[q] = q < nstage ? alpha[q] : 0.0;
+  a.nstage = nstage;
+  a.niter = niter;
+  a.linf_cap = c->linf_cap;
+  CoefB fm;
+  for (int b = 0; b < bes_ctx::kMaxBatch; b++)
+    fm.c[b] = (b < (int)c->lim_frozen.size() && c->lim_frozen[b]) ? 1.0 : 0.0;
+  const void *fn = c->nd == 2 ? (const void *)k_explicit_persistent<2> : (const void *)k_explicit_persistent<3>;
+  const int bs = 256;
+  const size_t lds = (size_t)bs * c->nv * (c->nd + 1) * sizeof(double);
+  int dev = 0, ncu = 0, per_cu = 0;
+  BES_HIP(hipGetDevice(&dev));
+  BES_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  BES_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, bs, lds));
+  if (per_cu < 1) { set_error("persistent explicit step: no resident block fits"); return BES_ERR_ARG; }
+  const int want = std::max(grid_for(c->N, bs), grid_for(c->dm.F, bs));
+  const int grid = std::max(1, std::min(per_cu * ncu, want));   // all blocks resident: the barriers complete
+  void *args[] = {&a, &c->dm, &c->gc, &fm};
+  BES_HIP(hipLaunchCooperativeKernel(fn, dim3(grid), dim3(bs), args, (unsigned)lds, c->stream));
+  if (nstage == 1 && niter % 2 == 1) std::swap(c->U0, c->Ua);
+  return BES_OK;
+}
+
+int bes_explicit_persistent(bes_ctx *c, int on) {
+  if (!c) { set_error("null context"); return BES_ERR_ARG; }
+  c->use_persistent = on != 0;
+  return BES_OK;
+}
+
 int bes_explicit_graph(bes_ctx *c, int on) {
   if (!c) { set_error("null context"); return BES_ERR_ARG; }
   c->use_graph = on != 0;
@@ -1216,8 +1363,11 @@ int bes_explicit_run(bes_ctx *c, int niter, int nstage, const double *alpha, dou
   int s;
   if ((s = reset_err(c))) return s;
   c->dt_local = local != 0;
+  const bool persist = c->use_persistent && c->B == 1 && local && !c->prof;
   const bool graph = c->use_graph && nstage > 1 && !c->prof;
-  if (graph) {
+  if (persist) {
+    if ((s = explicit_run_persistent(c, niter, nstage, alpha, cfl, linf_out != nullptr))) return s;
+  } else if (graph) {
     if ((s = capture_iteration(c, nstage, alpha, cfl, local, linf_out != nullptr))) return s;
     BES_HIP(hipMemsetAsync(c->dit, 0, sizeof(int), c->stream));
     for (int it = 0; it < niter; it++) BES_HIP(hipGraphLaunch(c->graph, c->stream));

@@ -97,8 +97,11 @@ __device__ __forceinline__ double block_min(double v) {
 // not one, so thousands of blocks do not queue on a few addresses; a reduce
 // over the copies follows (max is exact, so the order does not matter).
 constexpr int kLinfSlots = 64;
+__device__ __forceinline__ unsigned long long *linf_slot_of(unsigned long long *bits, int nq, int blk) {
+  return bits + (size_t)(blk & (kLinfSlots - 1)) * nq;
+}
 __device__ __forceinline__ unsigned long long *linf_slot(unsigned long long *bits, int nq) {
-  return bits + (size_t)(blockIdx.x & (kLinfSlots - 1)) * nq;
+  return linf_slot_of(bits, nq, blockIdx.x);
 }
 // out[q] = max over the copies of bits[s*nq + q] (out may be bits: copy 0)
 __device__ __forceinline__ unsigned long long linf_slots_max(const unsigned long long *bits, int nq, int q) {

@@ -152,6 +152,14 @@ int bes_explicit_run(bes_ctx *ctx, int niter, int nstage, const double *alpha, d
  * stage, the monitor) as a HIP graph and replays it niter times, one graph
  * launch per step (stage counts > 1, profiling off; no reference counterpart) */
 int bes_explicit_graph(bes_ctx *ctx, int on);
+/* on != 0: bes_explicit_run runs all niter steps as one cooperative launch of a
+ * persistent kernel whose resident blocks sweep each pass (time step; per
+ * stage gradient and limiter, face fluxes, residual and update) and meet at a
+ * grid barrier between passes; results bitwise those of the launched passes.
+ * Used for one condition with local time steps and profiling off, otherwise
+ * the launched (or graph) form runs; takes precedence over the graph (no
+ * reference counterpart; the reference's loop is SolverExplicit.cpp:395-438) */
+int bes_explicit_persistent(bes_ctx *ctx, int on);
 
 /* ---- Flux_::computeFlux / FluxJacobian::computeJacobian over a batch of faces
  *      (Flux.hpp:43-46, FluxJacobian.hpp:36-44; FluxRoe.tpp:306-333,
