@@ -18,6 +18,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "kernels_common.hpp"
+
 namespace bes {
 
 struct GasConst {
@@ -117,6 +119,39 @@ struct Gas {
   }
 };
 
+// Quotients by one divisor: for double the shared-reciprocal form (RcpDiv,
+// bitwise the division), for Dual the division itself.
+template <typename T>
+struct Divider {
+  T b;
+  __device__ __forceinline__ explicit Divider(const T &b_) : b(b_) {}
+  __device__ __forceinline__ T operator()(const T &a) const { return a / b; }
+};
+template <>
+struct Divider<double> {
+  RcpDiv r;
+  __device__ __forceinline__ explicit Divider(double b_) : r(b_) {}
+  __device__ __forceinline__ double operator()(double a) const { return r(a); }
+};
+
+// One side of the Roe flux: the primitive state, total enthalpy and normal
+// velocity, with the five divisions by rho of Gas::primitive, total_enthalpy
+// and contravariant_cons done by one Divider (the same values)
+template <int ND, typename T>
+__device__ __forceinline__ void roe_side(const Gas<ND, T> &g, const T *U, const double *n, T *P, T &H, T &V) {
+  constexpr int IE = ND + 1, IP = ND + 1, IT = ND + 2;
+  const Divider<T> by_rho(U[0]);
+  P[0] = U[0];
+#pragma unroll
+  for (int d = 0; d < ND; d++) P[1 + d] = by_rho(U[1 + d]);
+  P[IP] = g.pressure(U);
+  P[IT] = g.temperature(U);
+  H = by_rho(U[IE]) + by_rho(P[IP]);
+  V = 0.0;
+#pragma unroll
+  for (int d = 0; d < ND; d++) V += P[1 + d] * n[d];
+}
+
 // Harten entropy fix on an eigenvalue magnitude
 template <typename T>
 __device__ __forceinline__ T harten_fix(const T &lam, const T &delta) {
@@ -132,11 +167,9 @@ __device__ __forceinline__ void roe_flux(const GasConst &gc, const T *UL, const 
   Gas<ND, T> g(gc);
   // extended primitive states: [rho, u_d, p, T] + H, V (c and q^2 of the
   // sides enter only through the deltas, which the dissipation never uses)
-  T PL[ND + 3], PR[ND + 3];
-  g.primitive(UL, PL);
-  g.primitive(UR, PR);
-  T HL = g.total_enthalpy(UL), HR = g.total_enthalpy(UR);
-  T VL = g.contravariant_cons(UL, n), VR = g.contravariant_cons(UR, n);
+  T PL[ND + 3], PR[ND + 3], HL, HR, VL, VR;
+  roe_side<ND, T>(g, UL, n, PL, HL, VL);
+  roe_side<ND, T>(g, UR, n, PR, HR, VR);
   T dRho = PR[0] - PL[0], dP = PR[IP] - PL[IP], dV = VR - VL;
   T du[ND];
 #pragma unroll
@@ -144,16 +177,17 @@ __device__ __forceinline__ void roe_flux(const GasConst &gc, const T *UL, const 
   // Roe averages
   T sl = dsqrt(PL[0]), sr = dsqrt(PR[0]);
   T den = sl + sr;
+  const Divider<T> by_den(den);
   T aRho = dsqrt(PL[0] * PR[0]);
   T au[ND];
   T aQ = 0.0, aV = 0.0;
 #pragma unroll
   for (int d = 0; d < ND; d++) {
-    au[d] = (PL[1 + d] * dsqrt(PL[0]) + PR[1 + d] * dsqrt(PR[0])) / den;
+    au[d] = by_den(PL[1 + d] * dsqrt(PL[0]) + PR[1 + d] * dsqrt(PR[0]));
     aQ += au[d] * au[d];
     aV += au[d] * n[d];
   }
-  T aH = (HL * dsqrt(PL[0]) + HR * dsqrt(PR[0])) / den;
+  T aH = by_den(HL * dsqrt(PL[0]) + HR * dsqrt(PR[0]));
   T aC = dsqrt((T(gc.gamma) - 1.0) * (aH - aQ / 2.0));
   // central part
   T tmp[NV];
@@ -166,8 +200,9 @@ __device__ __forceinline__ void roe_flux(const GasConst &gc, const T *UL, const 
 #pragma unroll
   for (int i = 0; i < NV; i++) F[i] += tmp[i];
   T delta = dabs(aC * gc.harten);
+  const Divider<T> by_2c2(2.0 * aC * aC);
   {  // acoustic wave V - c
-    T cc = harten_fix(dabs(aV - aC), delta) * (dP - aRho * aC * dV) / (2.0 * aC * aC);
+    T cc = by_2c2(harten_fix(dabs(aV - aC), delta) * (dP - aRho * aC * dV));
     F[0] -= cc;
 #pragma unroll
     for (int d = 0; d < ND; d++) F[1 + d] -= cc * (au[d] - aC * n[d]);
@@ -187,7 +222,7 @@ __device__ __forceinline__ void roe_flux(const GasConst &gc, const T *UL, const 
     F[NV - 1] -= (c1 * aQ / 2.0 + c2 * (t - dV * aV));
   }
   {  // acoustic wave V + c
-    T cc = harten_fix(dabs(aV + aC), delta) * (dP + aRho * aC * dV) / (2.0 * aC * aC);
+    T cc = by_2c2(harten_fix(dabs(aV + aC), delta) * (dP + aRho * aC * dV));
     F[0] -= cc;
 #pragma unroll
     for (int d = 0; d < ND; d++) F[1 + d] -= cc * (au[d] + aC * n[d]);
