@@ -341,8 +341,10 @@ __global__ __launch_bounds__(256) void k_ilu_factor_color(const TM *__restrict__
     nb[m] = cnb[(size_t)m * N + p];
     np_[m] = cnp[(size_t)m * N + p];
   }
-  // the row's own blocks into the factor storage (boundary slots stay zero)
-#pragma unroll
+  // the row's own blocks into the factor storage (boundary slots stay zero),
+  // a block at a time (unrolled over the slots, the compiler would hold all
+  // 125 entries in registers between the loads and the stores)
+#pragma unroll 1
   for (int sl = 0; sl <= K; sl++) {
     if (sl > 0 && nb[sl - 1] < 0) continue;
 #pragma unroll
@@ -371,27 +373,38 @@ __global__ __launch_bounds__(256) void k_ilu_factor_color(const TM *__restrict__
     }
   double Aii[NB];
   load_block_b<TM, NB>(LU, 0, N, p, B, b, Aii);
+  // Each product below sums over pp in the same order as a whole-block
+  // product would; they are arranged so that few values are live at once (one
+  // row of A_ik, one column of U_kj), which keeps the kernel at two waves per
+  // SIMD instead of one.
   for (int q = 0; q < nl; q++) {
     const int m = ord[q];
     const int pk = np_[m];
-    double Lik[NB], Uinv[NB], Aik[NB];
-    load_block_b<TM, NB>(LU, 1 + m, N, p, B, b, Aik);
-    // row k is final: inv(U_kk), from the row-major copy (one contiguous block)
+    double Lik[NB];
+    {
+      // row k is final: inv(U_kk), from the row-major copy (one contiguous block)
+      double Uinv[NB];
 #pragma unroll
-    for (int e = 0; e < NB; e++) Uinv[e] = (double)Dr[((size_t)pk * NB + e) * B + b];
+      for (int e = 0; e < NB; e++) Uinv[e] = (double)Dr[((size_t)pk * NB + e) * B + b];
 #pragma unroll
-    for (int r = 0; r < NV; r++)
+      for (int r = 0; r < NV; r++) {
+        double ar[NV];   // row r of A_ik
 #pragma unroll
-      for (int c = 0; c < NV; c++) {
-        double s = 0;
+        for (int pp = 0; pp < NV; pp++) ar[pp] = (double)LU[(((size_t)(1 + m) * NB + r * NV + pp) * N + p) * B + b];
 #pragma unroll
-        for (int pp = 0; pp < NV; pp++) s += Aik[r * NV + pp] * Uinv[pp * NV + c];
-        Lik[r * NV + c] = s;
+        for (int c = 0; c < NV; c++) {
+          double s = 0;
+#pragma unroll
+          for (int pp = 0; pp < NV; pp++) s += ar[pp] * Uinv[pp * NV + c];
+          Lik[r * NV + c] = s;
+        }
       }
+    }
     store_block_b<TM, NB>(LU, 1 + m, N, p, B, b, Lik);
     // A_ij -= L_ik U_kj for j in row k's upper part that are in row i's pattern
+    // (one j at a time: unrolled, the U_kj loads of all j would be in flight)
     const unsigned lowk = clow[pk];
-#pragma unroll
+#pragma unroll 1
     for (int mk = 0; mk < K; mk++) {
       const int jk = cnb[(size_t)mk * N + pk];
       if (jk < 0 || ((lowk >> mk) & 1)) continue;   // not upper in row k
@@ -402,28 +415,24 @@ __global__ __launch_bounds__(256) void k_ilu_factor_color(const TM *__restrict__
         for (int mm = 0; mm < K; mm++) if (nb[mm] == jk) target = mm;
       }
       if (target == -1) continue;
-      double Ukj[NB];
+      const TM *urec = Ur + ((size_t)pk * K + mk) * NB * B + b;
 #pragma unroll
-      for (int e = 0; e < NB; e++) Ukj[e] = (double)Ur[(((size_t)pk * K + mk) * NB + e) * B + b];
-      double upd[NB];
+      for (int c = 0; c < NV; c++) {
+        double uc[NV];   // column c of U_kj
 #pragma unroll
-      for (int r = 0; r < NV; r++)
+        for (int pp = 0; pp < NV; pp++) uc[pp] = (double)urec[(size_t)(pp * NV + c) * B];
 #pragma unroll
-        for (int c = 0; c < NV; c++) {
+        for (int r = 0; r < NV; r++) {
           double s = 0;
 #pragma unroll
-          for (int pp = 0; pp < NV; pp++) s += Lik[r * NV + pp] * Ukj[pp * NV + c];
-          upd[r * NV + c] = s;
+          for (int pp = 0; pp < NV; pp++) s += Lik[r * NV + pp] * uc[pp];
+          if (target == -2) {
+            Aii[r * NV + c] -= s;
+          } else {
+            const size_t qa = (((size_t)(1 + target) * NB + r * NV + c) * N + p) * B + b;
+            LU[qa] = (TM)((double)LU[qa] - s);
+          }
         }
-      if (target == -2) {
-#pragma unroll
-        for (int e = 0; e < NB; e++) Aii[e] -= upd[e];
-      } else {
-        double Aij[NB];
-        load_block_b<TM, NB>(LU, 1 + target, N, p, B, b, Aij);
-#pragma unroll
-        for (int e = 0; e < NB; e++) Aij[e] -= upd[e];
-        store_block_b<TM, NB>(LU, 1 + target, N, p, B, b, Aij);
       }
     }
   }
@@ -433,7 +442,8 @@ __global__ __launch_bounds__(256) void k_ilu_factor_color(const TM *__restrict__
 #pragma unroll
   for (int e = 0; e < NB; e++) Dr[((size_t)p * NB + e) * B + b] = (TM)Inv[e];
   // upper blocks, final from here on, row-major for the rows of later colours
-#pragma unroll
+  // (a block at a time, as the copy above)
+#pragma unroll 1
   for (int m = 0; m < K; m++) {
     if (nb[m] < 0 || ((lowm >> m) & 1)) continue;
 #pragma unroll

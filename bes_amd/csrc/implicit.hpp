@@ -131,11 +131,21 @@ __device__ __forceinline__ void block_inverse(double *A, double *Inv) {
       double a = ::fabs(A[r * NV + p]);
       if (a > best) { best = a; piv = r; }
     }
-    if (piv != p) {
+    // rows p and piv swap; written as a conditional exchange with every row
+    // below p so that all indices are compile-time constants and A and Inv
+    // stay in registers (an exchange through a run-time row index put both
+    // arrays in per-lane scratch memory for nv = 5)
+#pragma unroll
+    for (int r = p + 1; r < NV; r++) {
+      const bool sw = piv == r;
 #pragma unroll
       for (int c = 0; c < NV; c++) {
-        double t = A[p * NV + c]; A[p * NV + c] = A[piv * NV + c]; A[piv * NV + c] = t;
-        t = Inv[p * NV + c]; Inv[p * NV + c] = Inv[piv * NV + c]; Inv[piv * NV + c] = t;
+        const double a = A[p * NV + c], b = A[r * NV + c];
+        A[p * NV + c] = sw ? b : a;
+        A[r * NV + c] = sw ? a : b;
+        const double x = Inv[p * NV + c], y = Inv[r * NV + c];
+        Inv[p * NV + c] = sw ? y : x;
+        Inv[r * NV + c] = sw ? x : y;
       }
     }
     const double inv = 1.0 / A[p * NV + p];
