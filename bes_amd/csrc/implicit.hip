@@ -270,6 +270,10 @@ __global__ __launch_bounds__(256) void k_spmv_pos(const TM *__restrict__ A, cons
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int p = t >> lgB, b = t & (B - 1);
   if (p >= N) return;
+  // block entries from the row's first entry with the uniform stride es (a
+  // full index per entry costs a 64-bit register pair per load in flight)
+  const size_t es = (size_t)N * B;
+  const TM *Ar = A + (size_t)p * B + b;
   double acc[NV], xv[NV];
 #pragma unroll
   for (int c = 0; c < NV; c++) xv[c] = (double)x[((size_t)p * NV + c) * B + b];
@@ -277,7 +281,7 @@ __global__ __launch_bounds__(256) void k_spmv_pos(const TM *__restrict__ A, cons
   for (int r = 0; r < NV; r++) {
     double s = 0;
 #pragma unroll
-    for (int c = 0; c < NV; c++) s += (double)A[((size_t)(r * NV + c) * N + p) * B + b] * xv[c];
+    for (int c = 0; c < NV; c++) s += (double)Ar[(r * NV + c) * es] * xv[c];
     acc[r] = s;
   }
 #pragma unroll
@@ -286,12 +290,12 @@ __global__ __launch_bounds__(256) void k_spmv_pos(const TM *__restrict__ A, cons
     if (q < 0) continue;
 #pragma unroll
     for (int c = 0; c < NV; c++) xv[c] = (double)x[((size_t)q * NV + c) * B + b];
-    const TM *Ao = A + (size_t)(1 + m) * NB * N * B;
+    const TM *Ao = Ar + (size_t)(1 + m) * NB * es;
 #pragma unroll
     for (int r = 0; r < NV; r++) {
       double s = 0;
 #pragma unroll
-      for (int c = 0; c < NV; c++) s += (double)Ao[((size_t)(r * NV + c) * N + p) * B + b] * xv[c];
+      for (int c = 0; c < NV; c++) s += (double)Ao[(r * NV + c) * es] * xv[c];
       acc[r] += s;
     }
   }
@@ -373,6 +377,7 @@ __global__ __launch_bounds__(256) void k_ilu_factor_color(const TM *__restrict__
     }
   double Aii[NB];
   load_block_b<TM, NB>(LU, 0, N, p, B, b, Aii);
+  const size_t es = (size_t)N * B;   // stride between a block's entries
   // Each product below sums over pp in the same order as a whole-block
   // product would; they are arranged so that few values are live at once (one
   // row of A_ik, one column of U_kj), which keeps the kernel at two waves per
@@ -386,11 +391,14 @@ __global__ __launch_bounds__(256) void k_ilu_factor_color(const TM *__restrict__
       double Uinv[NB];
 #pragma unroll
       for (int e = 0; e < NB; e++) Uinv[e] = (double)Dr[((size_t)pk * NB + e) * B + b];
+      // (entries addressed from a per-row base with a uniform stride: computing
+      // each entry's full index costs a 64-bit register pair per load in flight)
+      const TM *aik = LU + ((size_t)(1 + m) * NB * N + p) * B + b;
 #pragma unroll
       for (int r = 0; r < NV; r++) {
         double ar[NV];   // row r of A_ik
 #pragma unroll
-        for (int pp = 0; pp < NV; pp++) ar[pp] = (double)LU[(((size_t)(1 + m) * NB + r * NV + pp) * N + p) * B + b];
+        for (int pp = 0; pp < NV; pp++) ar[pp] = (double)aik[(r * NV + pp) * es];
 #pragma unroll
         for (int c = 0; c < NV; c++) {
           double s = 0;
@@ -429,8 +437,8 @@ __global__ __launch_bounds__(256) void k_ilu_factor_color(const TM *__restrict__
           if (target == -2) {
             Aii[r * NV + c] -= s;
           } else {
-            const size_t qa = (((size_t)(1 + target) * NB + r * NV + c) * N + p) * B + b;
-            LU[qa] = (TM)((double)LU[qa] - s);
+            TM *aij = LU + ((size_t)(1 + target) * NB * N + p) * B + b + (r * NV + c) * es;
+            *aij = (TM)((double)*aij - s);
           }
         }
       }
@@ -468,6 +476,8 @@ __global__ __launch_bounds__(256) void k_ilu_lower(const TM *__restrict__ LU, in
   if (t >= (nrows << lgB)) return;
   const int p = p0 + (t >> lgB), b = t & (B - 1);
   const unsigned lowm = clow[p];
+  const size_t es = (size_t)N * B;   // stride between a block's entries
+  const TM *Lr = LU + (size_t)p * B + b;
   double acc[NV];
 #pragma unroll
   for (int c = 0; c < NV; c++) acc[c] = (double)r[((size_t)p * NV + c) * B + b];
@@ -479,12 +489,12 @@ __global__ __launch_bounds__(256) void k_ilu_lower(const TM *__restrict__ LU, in
     double yk[NV];
 #pragma unroll
     for (int c = 0; c < NV; c++) yk[c] = (double)src[((size_t)q * NV + c) * B + b];
-    const TM *L = LU + (size_t)(1 + m) * NB * N * B;
+    const TM *L = Lr + (size_t)(1 + m) * NB * es;
 #pragma unroll
     for (int a = 0; a < NV; a++) {
       double s = 0;
 #pragma unroll
-      for (int c = 0; c < NV; c++) s += (double)L[((size_t)(a * NV + c) * N + p) * B + b] * yk[c];
+      for (int c = 0; c < NV; c++) s += (double)L[(a * NV + c) * es] * yk[c];
       acc[a] -= s;
     }
   }
@@ -495,7 +505,7 @@ __global__ __launch_bounds__(256) void k_ilu_lower(const TM *__restrict__ LU, in
     for (int a = 0; a < NV; a++) {
       double s = 0;
 #pragma unroll
-      for (int c = 0; c < NV; c++) s += (double)LU[((size_t)(a * NV + c) * N + p) * B + b] * acc[c];
+      for (int c = 0; c < NV; c++) s += (double)Lr[(a * NV + c) * es] * acc[c];
       x[((size_t)p * NV + a) * B + b] = (TV)s;
     }
   } else {
@@ -518,6 +528,8 @@ __global__ __launch_bounds__(256) void k_ilu_upper(const TM *__restrict__ LU, in
   const int p = p0 + (t >> lgB), b = t & (B - 1);
   const unsigned lowm = clow[p];
   const TV *own = p < y0end ? r : y;
+  const size_t es = (size_t)N * B;   // stride between a block's entries
+  const TM *Lr = LU + (size_t)p * B + b;
   double acc[NV];
 #pragma unroll
   for (int c = 0; c < NV; c++) acc[c] = (double)own[((size_t)p * NV + c) * B + b];
@@ -528,12 +540,12 @@ __global__ __launch_bounds__(256) void k_ilu_upper(const TM *__restrict__ LU, in
     double xj[NV];
 #pragma unroll
     for (int c = 0; c < NV; c++) xj[c] = (double)x[((size_t)q * NV + c) * B + b];
-    const TM *Uo = LU + (size_t)(1 + m) * NB * N * B;
+    const TM *Uo = Lr + (size_t)(1 + m) * NB * es;
 #pragma unroll
     for (int a = 0; a < NV; a++) {
       double s = 0;
 #pragma unroll
-      for (int c = 0; c < NV; c++) s += (double)Uo[((size_t)(a * NV + c) * N + p) * B + b] * xj[c];
+      for (int c = 0; c < NV; c++) s += (double)Uo[(a * NV + c) * es] * xj[c];
       acc[a] -= s;
     }
   }
@@ -541,7 +553,7 @@ __global__ __launch_bounds__(256) void k_ilu_upper(const TM *__restrict__ LU, in
   for (int a = 0; a < NV; a++) {
     double s = 0;
 #pragma unroll
-    for (int c = 0; c < NV; c++) s += (double)LU[((size_t)(a * NV + c) * N + p) * B + b] * acc[c];
+    for (int c = 0; c < NV; c++) s += (double)Lr[(a * NV + c) * es] * acc[c];
     x[((size_t)p * NV + a) * B + b] = (TV)s;
   }
 }

@@ -176,17 +176,23 @@ __device__ __forceinline__ void store_block(TM *__restrict__ base, int slot, int
 }
 // the same for a batch of conditions (condition b fastest): entry e of slot s
 // of row i at ((s*NB + e)*N + i)*nb + b; nb = 1 gives the layouts above
+// (addressed from the block's first entry with a uniform stride: a full index
+// per entry costs a 64-bit register pair per access in flight)
 template <typename TM, int NB>
 __device__ __forceinline__ void load_block_b(const TM *__restrict__ base, int slot, int N, int i, int nb, int b,
                                              double *B) {
+  const TM *q = base + ((size_t)slot * NB * N + i) * nb + b;
+  const size_t es = (size_t)N * nb;
 #pragma unroll
-  for (int e = 0; e < NB; e++) B[e] = (double)base[(((size_t)slot * NB + e) * N + i) * nb + b];
+  for (int e = 0; e < NB; e++) B[e] = (double)q[e * es];
 }
 template <typename TM, int NB>
 __device__ __forceinline__ void store_block_b(TM *__restrict__ base, int slot, int N, int i, int nb, int b,
                                               const double *B) {
+  TM *q = base + ((size_t)slot * NB * N + i) * nb + b;
+  const size_t es = (size_t)N * nb;
 #pragma unroll
-  for (int e = 0; e < NB; e++) base[(((size_t)slot * NB + e) * N + i) * nb + b] = (TM)B[e];
+  for (int e = 0; e < NB; e++) q[e * es] = (TM)B[e];
 }
 
 __device__ __forceinline__ double to_d(double x) { return x; }
