@@ -260,7 +260,8 @@ __device__ __forceinline__ void gradlim_cell(const double *__restrict__ U, const
 // Writes the cell pack, per variable the group [U_v | phi_v * grad U_v] at
 // ((i*NV + v)*GW + q)*B + b, GW = 1 + nd.  (One wave per variable instead
 // keeps fewer registers per thread but repeats the geometry loads and address
-// arithmetic per variable: 427 against 355 us on the 2M-tet box.)
+// arithmetic per variable; on the 2M-tet box it was the slower form, 427
+// against 355 us, when both stored their packs per thread.)
 // With frz non-null (limiter freeze on): a condition b with fmask.c[b] != 0
 // takes its limiter from frz, every other condition stores its limiter there.
 // With one condition the packs of the block's cells (contiguous in PK) are
@@ -517,8 +518,10 @@ __global__ void k_linf_reduce(unsigned long long *bits, int nq) {
 // fluxes, residual and update) separated by grid barriers, each pass a loop
 // of the grid's blocks over the pass's blocks of 256 cells or faces.  Every
 // value is computed by the same device code as the per-pass kernels, so the
-// states and monitors are bitwise those of the launched form; what goes is
-// the launch per pass (14 per RK4 step), which dominates on small meshes.
+// states and monitors are bitwise those of the launched form.  A grid barrier
+// replaces each launch (14 per RK4 step); on one MI355X the barrier costs more
+// than a graph-launched kernel on a 40k-cell mesh, so the graph stays the
+// default and this form is an option.
 struct PersistStep {
   double *U0, *Ua, *Ub;   // state slots as in explicit_iteration
   double *PK, *FF, *dt, *frz, *linf;
