@@ -21,15 +21,16 @@
 // Loss: K steps of preconditioned Richardson from probe vectors v (the
 // right-hand side and seeded Gaussian vectors), L = mean ||r_K||^2/||v||^2.
 //
-// Kernels.  Every dense contraction runs on the matrix cores as
-// V_MFMA_F32_32X32X8_BF16 tiles (bf16 operands, fp32 accumulation): the
+// Kernels.  Every dense contraction runs on the matrix cores as gfx950's
+// V_MFMA_F32_32X32X16_BF16 tiles (bf16 operands, fp32 accumulation): the
 // forward layers, the back-propagated products dX = dY W and the weight
 // gradients dW = dY^T X (rows of the tile as the reduction dimension).  One
 // wave owns 32 cells: a node tile is 32 cells, an edge tile 8 cells x 4 face
 // slots.  Lane l of a tile supplies row l%32; lane half h = l/32 supplies
-// operand columns [K/2*h, K/2*h + K/2) in 4-element steps, the same split for
-// both operands, so a lane reads its row (or weight row) as one contiguous
-// run.  Output element q of lane l is row 8*(q/4) + 4*h + q%4, column l%32.
+// operand columns [K/2*h, K/2*h + K/2) in 4-element runs (two runs per
+// instruction), the same split for both operands, so a lane reads its row (or
+// weight row) as one contiguous run.  Output element q of lane l is row
+// 8*(q/4) + 4*h + q%4, column l%32.
 // Weight gradients are reduced per wave into a partial buffer and summed in a
 // fixed order (deterministic).  Adam (Kingma & Ba 2015) updates fp32 master
 // weights and refreshes bf16 copies, plain and transposed.
@@ -63,11 +64,9 @@ constexpr int kTrainWaves = kTrainBlocks * 4;
 constexpr int kDotBlocks = 512;
 
 typedef short bf4 __attribute__((ext_vector_type(4)));
+typedef short bf8s __attribute__((ext_vector_type(8)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 typedef float f16v __attribute__((ext_vector_type(16)));
-
-__device__ __forceinline__ f16v mma(bf4 a, bf4 b, f16v c) {
-  return __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, c, 0, 0, 0);
-}
 
 // float -> bfloat16, round to nearest even
 __host__ __device__ __forceinline__ uint16_t f2bf(float f) {
@@ -104,10 +103,21 @@ __device__ __forceinline__ f16v zero16() {
   return z;
 }
 
+// KS 4-element operand runs per lane (run s of A and run s of B hold the
+// same k indices for every lane) through the 32x32x16 BF16 MFMA, two runs per
+// instruction: a lane's 8-element operand is its runs s and s+1 back to back,
+// in A and in B alike, so every product pairs the same k of A and B and each
+// instruction sums the k of both runs (in its own order)
 template <int KS>
 __device__ __forceinline__ f16v mma_run(const bf4 *a, const bf4 *b, f16v acc) {
+  static_assert(KS % 2 == 0, "operand runs are consumed in pairs");
 #pragma unroll
-  for (int s = 0; s < KS; s++) acc = mma(a[s], b[s], acc);
+  for (int s = 0; s < KS; s += 2) {
+    const bf8s sa = __builtin_shufflevector(a[s], a[s + 1], 0, 1, 2, 3, 4, 5, 6, 7);
+    const bf8s sb = __builtin_shufflevector(b[s], b[s + 1], 0, 1, 2, 3, 4, 5, 6, 7);
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf8, sa), __builtin_bit_cast(bf8, sb), acc, 0,
+                                                  0, 0);
+  }
   return acc;
 }
 
