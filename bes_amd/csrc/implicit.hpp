@@ -212,6 +212,9 @@ __global__ __launch_bounds__(256) void k_spmv(const TM *__restrict__ A, DevMesh 
   const int t = cell_index();
   const int i = t >> lgB, b = t & (B - 1);
   if (i >= N) return;
+  // block entries from the row's first entry with the uniform stride es
+  const size_t es = (size_t)N * B;
+  const TM *Ar = A + (size_t)i * B + b;
   double acc[NV];
   double xi[NV];
 #pragma unroll
@@ -220,7 +223,7 @@ __global__ __launch_bounds__(256) void k_spmv(const TM *__restrict__ A, DevMesh 
   for (int r = 0; r < NV; r++) {
     double s = 0;
 #pragma unroll
-    for (int c = 0; c < NV; c++) s += to_d(A[((size_t)(r * NV + c) * N + i) * B + b]) * xi[c];
+    for (int c = 0; c < NV; c++) s += to_d(Ar[(r * NV + c) * es]) * xi[c];
     acc[r] = s;
   }
 #pragma unroll
@@ -230,12 +233,12 @@ __global__ __launch_bounds__(256) void k_spmv(const TM *__restrict__ A, DevMesh 
     double xj[NV];
 #pragma unroll
     for (int c = 0; c < NV; c++) xj[c] = (double)x[((size_t)j * NV + c) * B + b];
-    const TM *Ao = A + (size_t)(1 + m) * NB * N * B;
+    const TM *Ao = Ar + (size_t)(1 + m) * NB * es;
 #pragma unroll
     for (int r = 0; r < NV; r++) {
       double s = 0;
 #pragma unroll
-      for (int c = 0; c < NV; c++) s += to_d(Ao[((size_t)(r * NV + c) * N + i) * B + b]) * xj[c];
+      for (int c = 0; c < NV; c++) s += to_d(Ao[(r * NV + c) * es]) * xj[c];
       acc[r] += s;
     }
   }
