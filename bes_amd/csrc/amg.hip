@@ -789,18 +789,44 @@ __global__ __launch_bounds__(256) void k_csr_group(const int *__restrict__ rp, c
   const int l = threadIdx.x & 31, gbase = threadIdx.x & 32;   // lane in group, group's first lane in the wave
   const bool act = l < NB;
   const int r = act ? l / NV : 0, cc = act ? l - r * NV : 0;
+  // blocks whose loads are in flight together: 8 or 16 hold more registers
+  // and gave slower V-cycles (5 waves per SIMD instead of 8).  Neither a lane
+  // per block row (12 rows a wave, 4 waves per SIMD) nor the row's entries
+  // dealt over a whole wave was faster on the 300k-row level of the 2M wing
+  constexpr int U = 4;
+  // the row's own data, loaded first so that its latency overlaps the sweep's
+  double fr = 0, xr = 0, wl = 0;
+  if (i < n && act) {
+    if (cc == 0) fr = (double)f[(size_t)i * NV + r];
+    if (RELAX && cc == 0) xr = (double)x[(size_t)i * NV + r];
+    if (RELAX && KIND == kBlockJacobi) wl = to_d(w[(size_t)i * NB + l]);
+    if (RELAX && KIND != kBlockJacobi && cc == 0) wl = to_d(w[(size_t)i * NV + r]);
+  }
   double acc = 0;
   if (i < n) {
     // the row's columns come in 32 at a time (one per lane) and reach the
-    // group by shuffles, so the block and x loads of a chunk are independent
+    // group by shuffles; the block and x loads of U columns are issued before
+    // any of them is used, and the products are summed in column order
     const int p0 = rp[i], len = rp[i + 1] - p0;
     for (int b0 = 0; b0 < len; b0 += 32) {
       const int nb = min(32, len - b0);
       const int Jl = l < nb ? cl[p0 + b0 + l] : 0;
-#pragma unroll 4
-      for (int q = 0; q < nb; q++) {
-        const int J = __shfl(Jl, gbase + q, 64);
-        if (act) acc += to_d(val[(size_t)(p0 + b0 + q) * NB + l]) * (double)x[(size_t)J * NV + cc];
+      for (int q0 = 0; q0 < nb; q0 += U) {
+        TS vq[U];
+        TV xq[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+          const int J = __shfl(Jl, gbase + ((q0 + u) & 31), 64);
+          vq[u] = 0;
+          xq[u] = 0;
+          if (act && q0 + u < nb) {
+            vq[u] = val[(size_t)(p0 + b0 + q0 + u) * NB + l];
+            xq[u] = x[(size_t)J * NV + cc];
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++)
+          if (act && q0 + u < nb) acc += to_d(vq[u]) * (double)xq[u];
       }
     }
   }
@@ -809,7 +835,7 @@ __global__ __launch_bounds__(256) void k_csr_group(const int *__restrict__ rp, c
   for (int c = 0; c < NV; c++) s += __shfl(acc, gbase + r * NV + c, 64);
   if (rm && i < n) s *= (double)rm[i];
   double t = 0;
-  if (i < n && act && cc == 0) t = (double)f[(size_t)i * NV + r] - s;   // lane r*NV: row r of f - A x
+  if (i < n && act && cc == 0) t = fr - s;   // lane r*NV: row r of f - A x
   if (RELAX == 0) {
     if (i < n && act && cc == 0) out[(size_t)i * NV + r] = (TV)t;
     return;
@@ -818,13 +844,13 @@ __global__ __launch_bounds__(256) void k_csr_group(const int *__restrict__ rp, c
   if (KIND == kBlockJacobi) {
     // lane r*NV + c: W[r][c] * t_c, t_c from lane c*NV, then the row sum
     const double tc = __shfl(t, gbase + cc * NV, 64);
-    const double prod = (i < n && act) ? to_d(w[(size_t)i * NB + l]) * tc : 0.0;
+    const double prod = (i < n && act) ? wl * tc : 0.0;
     d = 0;
     for (int c = 0; c < NV; c++) d += __shfl(prod, gbase + r * NV + c, 64);
   } else {
-    d = (i < n && act && cc == 0) ? to_d(w[(size_t)i * NV + r]) * t : 0.0;
+    d = (i < n && act && cc == 0) ? wl * t : 0.0;
   }
-  if (i < n && act && cc == 0) out[(size_t)i * NV + r] = (TV)((double)x[(size_t)i * NV + r] + d);
+  if (i < n && act && cc == 0) out[(size_t)i * NV + r] = (TV)(xr + d);
 }
 
 // fc[I] = sum of t over the members of I: G lanes per coarse row (G a power
