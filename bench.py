@@ -129,7 +129,7 @@ def implicit_workload(args):
 
         def setup(precision, precond, params=None):
             kind = "learned" if precond.startswith("learned") else precond
-            rtol = args.amg_rtol if kind == "amg" else args.rtol
+            rtol = args.amg_rtol if kind == "amg" else args.lp_rtol if kind == "learned" else args.rtol
             c.implicit_setup(precision, kind, restart=args.restart, maxiter=args.krylov_max, rtol=rtol)
             c.implicit_set_reorth(args.reorth)
             if kind in ("ilu0", "learned") and tuned["coloring"]:   # the ILU(0) fallback of the learned form too
@@ -920,6 +920,8 @@ def main():
                     help="form of the K timed implicit steps (the headline value is the plain form's)")
     ap.add_argument("--lp-rebuild", type=int, default=6,
                     help="learned operator rebuilt at every n-th solve of the mixed learned run")
+    ap.add_argument("--lp-rtol", type=float, default=5e-2,
+                    help="GMRES relative tolerance of the learned forms")
     ap.add_argument("--amg-rebuild", type=int, default=10,
                     help="AMG hierarchy rebuilt at every n-th solve of the mixed AMG run")
     ap.add_argument("--amg-rtol", type=float, default=0.15,
